@@ -1,0 +1,111 @@
+/*
+ * mrmath_gpu.h -- C ABI of libmrmath_gpu.so, the gfx950 (MI355X) backend for
+ * mrMath's dense FP64 multiply and LU/solve path.
+ *
+ * Conventions are the reference's (src/MatrixConst.pas, src/Matrix.pas):
+ * matrices are row-major, "width" is the number of columns, "height" the
+ * number of rows, every LineWidth is the row stride in BYTES (a multiple of 8),
+ * and sizes are 64-bit (NativeInt).  Enumerations are passed as int32.
+ *
+ * Two families of entry points:
+ *   - host-pointer entries with exactly the reference's function-pointer
+ *     signatures, for installation into the stub switch (they stage the
+ *     operands through device memory and return when the result is back in
+ *     host memory);
+ *   - device-pointer entries (mrm_dev_*) that enqueue on a caller-supplied HIP
+ *     stream (passed as void*, NULL = the library's stream) and return at once.
+ *
+ * Errors: the reference's GEMM procedures return nothing, so the GEMM entries
+ * record failures in a thread-local status read with mrm_last_error() /
+ * mrm_last_error_message().  LU / solve entries return the status directly.
+ * Every entry is safe to call from several host threads at once.
+ */
+#ifndef MRMATH_GPU_H
+#define MRMATH_GPU_H
+
+#include <stdint.h>
+
+#ifndef MRM_API
+#define MRM_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. MRM_SINGULAR equals the reference's leSingular. */
+#define MRM_OK 0
+#define MRM_SINGULAR 1
+#define MRM_BAD_DIMS 2
+#define MRM_BAD_ARG 3
+#define MRM_HIP_ERROR 4
+#define MRM_NO_MEMORY 5
+#define MRM_NOT_INITIALISED 6
+#define MRM_NOT_CONVERGED 7
+
+/* TMatrixMultDestOperation (reference src/MatrixConst.pas:147). */
+#define MRM_DO_NONE 0
+#define MRM_DO_ADD 1
+#define MRM_DO_SUB 2
+
+/* Operand layouts for the device entries. */
+#define MRM_NN 0 /* C = A * B   */
+#define MRM_TN 1 /* C = A^T * B (MultT1) */
+#define MRM_NT 2 /* C = A * B^T (MultT2) */
+
+/* ---- library state ------------------------------------------------------ */
+
+/* Selects the current HIP device, creates the library stream.  Called
+ * implicitly by the first entry; replaces nothing in the reference (its
+ * kernels are bound at unit initialisation, src/MatrixASMStubSwitch.pas:2080-2084). */
+MRM_API int32_t mrm_init(void);
+MRM_API void mrm_shutdown(void);
+MRM_API int32_t mrm_last_error(void);
+MRM_API const char* mrm_last_error_message(void);
+/* Library version, e.g. 0x000100 = 0.1.0. */
+MRM_API int32_t mrm_version(void);
+
+/* ---- host-pointer GEMM entries (stub-switch surface) --------------------- */
+
+/* TMatrixMultFunc (src/MatrixASMStubSwitch.pas:351), the type of multFunc:
+ * dest(height1 x width2) = mt1(height1 x width1) * mt2(height2 x width2),
+ * width1 = height2.  Replaces FMAMatrixMult (src/FMAMatrixOperations.pas:94-143). */
+MRM_API void mrm_mult(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
+              int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2);
+
+/* TMatrixMultFunc, the type of multT2Func: dest(height1 x height2) = mt1 * mt2^T,
+ * width1 = width2.  Replaces FMAMatrixMultTransposed (src/FMAMatrixOperations.pas:55-92). */
+MRM_API void mrm_mult_t2(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
+                 int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2);
+
+/* TMatrixBlockedMultfunc (src/MatrixASMStubSwitch.pas:352), blockedMultFunc:
+ * dest (op)= mt1 * mt2.  Replaces BlockMatrixMultiplication (src/BlockedMult.pas:350-474).
+ * blockSize and mem are accepted for ABI compatibility and ignored. */
+MRM_API void mrm_blocked_mult(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
+                      int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2,
+                      int64_t blockSize, int32_t op, double* mem);
+
+/* blockedMultT1Func: dest(width1 x width2) (op)= mt1^T * mt2, height1 = height2.
+ * Replaces BlockMatrixMultiplicationT1 (src/BlockedMult.pas:477-593). */
+MRM_API void mrm_blocked_mult_t1(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2,
+                         int64_t width1, int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1,
+                         int64_t LineWidth2, int64_t blockSize, int32_t op, double* mem);
+
+/* blockedMultT2Func: dest(height1 x height2) (op)= mt1 * mt2^T, width1 = width2.
+ * Replaces BlockMatrixMultiplicationT2 (src/BlockedMult.pas:596-724). */
+MRM_API void mrm_blocked_mult_t2(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2,
+                         int64_t width1, int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1,
+                         int64_t LineWidth2, int64_t blockSize, int32_t op, double* mem);
+
+/* ---- device-pointer GEMM -------------------------------------------------- */
+
+/* C(M x N) (op)= opA(A) * opB(B) with trans = MRM_NN / MRM_TN / MRM_NT, all
+ * pointers in device memory, strides in bytes.  The kernel behind every
+ * GEMM entry above; no reference counterpart (device-resident callers). */
+MRM_API int32_t mrm_dev_dgemm(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+"""ctypes binding of libmrmath_gpu.so (the C ABI declared in include/mrmath_gpu.h).
+
+The library is loaded from the package directory only; there is no fallback:
+if it is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmrmath_gpu.so"
+
+MRM_OK, MRM_SINGULAR, MRM_BAD_DIMS, MRM_BAD_ARG, MRM_HIP_ERROR, MRM_NO_MEMORY, MRM_NOT_INITIALISED, MRM_NOT_CONVERGED = range(8)
+DO_NONE, DO_ADD, DO_SUB = 0, 1, 2
+NN, TN, NT = 0, 1, 2
+
+_i64 = C.c_int64
+_i32 = C.c_int32
+_pd = C.c_void_p  # double*: passed as raw addresses
+_pv = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mrm_init": (_i32, []),
+    "mrm_shutdown": (None, []),
+    "mrm_last_error": (_i32, []),
+    "mrm_last_error_message": (C.c_char_p, []),
+    "mrm_version": (_i32, []),
+    "mrm_mult": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64]),
+    "mrm_mult_t2": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64]),
+    "mrm_blocked_mult": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _pd]),
+    "mrm_blocked_mult_t1": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _pd]),
+    "mrm_blocked_mult_t2": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _pd]),
+    "mrm_dev_dgemm": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _pv]),
+}
+
+_lib = None
+
+
+class MrmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mrmath_gpu error {code}: {msg}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    # torch (when present) bundles a HIP runtime with the same soname as ROCm's;
+    # loading it first makes the library bind to that one runtime instead of a
+    # second copy, so device pointers from torch tensors are valid here.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m mrmath_amd.build` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code: int | None = None) -> None:
+    """Raises MrmError for a non-zero status (the given one or the thread's last)."""
+    lib = load()
+    if code is None:
+        code = lib.mrm_last_error()
+    if code != MRM_OK:
+        msg = lib.mrm_last_error_message()
+        raise MrmError(code, msg.decode() if msg else "")

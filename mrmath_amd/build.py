@@ -1,0 +1,85 @@
+"""Builds libmrmath_gpu.so (hand-written HIP for gfx950) in-tree.
+
+Every source under ``mrmath_amd/csrc`` is compiled with hipcc for gfx950 and
+linked into ``mrmath_amd/libmrmath_gpu.so``.  Objects are rebuilt only when a
+source or header is newer; ``force=True`` rebuilds everything.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OBJ = PKG / "_obj"
+LIB = PKG / "libmrmath_gpu.so"
+INCLUDE = PKG.parent / "include"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+CFLAGS = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-fvisibility=hidden",
+    f"--offload-arch={ARCH}",
+    # no packed FP32 instructions in any generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    f"-I{CSRC}",
+    f"-I{INCLUDE}",
+]
+
+
+def _sources():
+    return sorted(CSRC.glob("*.hip"))
+
+
+def _needs(obj: Path, src: Path, headers) -> bool:
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    return src.stat().st_mtime > t or any(h.stat().st_mtime > t for h in headers)
+
+
+def _compile(src: Path, obj: Path):
+    cmd = [HIPCC, *CFLAGS, "-c", str(src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+    return src.name
+
+
+def build(force: bool = False, verbose: bool = True) -> Path:
+    OBJ.mkdir(exist_ok=True)
+    headers = list(CSRC.glob("*.h")) + list(INCLUDE.glob("*.h"))
+    jobs = []
+    objs = []
+    for src in _sources():
+        obj = OBJ / (src.stem + ".o")
+        objs.append(obj)
+        if force or _needs(obj, src, headers):
+            jobs.append((src, obj))
+    if jobs:
+        workers = min(len(jobs), max(1, min(8, os.cpu_count() or 1)))
+        with cf.ThreadPoolExecutor(workers) as ex:
+            for name in ex.map(lambda j: _compile(*j), jobs):
+                if verbose:
+                    print(f"[mrmath_amd] compiled {name}", file=sys.stderr)
+    if force or jobs or not LIB.exists():
+        tmp = LIB.with_suffix(".so.tmp")
+        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, LIB)
+        if verbose:
+            print(f"[mrmath_amd] linked {LIB}", file=sys.stderr)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(force="--force" in sys.argv)

@@ -1,0 +1,27 @@
+// mrm_common.h -- shared definitions for the gfx950 kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double mrm_d4 __attribute__((ext_vector_type(4)));
+typedef float mrm_f4 __attribute__((ext_vector_type(4)));
+
+// Status codes, MRM_NN/TN/NT and MRM_DO_* come from the public header.
+#include "mrmath_gpu.h"
+
+// Internal launchers (all device pointers, byte strides, enqueue on `stream`).
+int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                     const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream);
+
+void mrm_set_error(int code, const char* fmt, ...);
+
+#define MRM_HIP_TRY(expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            mrm_set_error(MRM_HIP_ERROR, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                          __FILE__, __LINE__);                                             \
+            return MRM_HIP_ERROR;                                                          \
+        }                                                                                  \
+    } while (0)

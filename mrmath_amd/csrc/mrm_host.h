@@ -1,0 +1,16 @@
+// mrm_host.h -- host-side helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <mutex>
+
+#include "mrm_common.h"
+
+// Staging slots for host-pointer entries: 0 = A, 1 = B, 2 = C/X, 3.. = extra.
+constexpr int MRM_NUM_SLOTS = 8;
+
+int mrm_ensure_init();
+hipStream_t mrm_stream();
+hipStream_t mrm_resolve_stream(void* s);
+std::mutex& mrm_host_mutex();
+// Grow-only device buffer for `slot`; callers hold mrm_host_mutex().
+int mrm_staging(int slot, size_t bytes, void** out);

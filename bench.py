@@ -156,7 +156,7 @@ def main():
         "data": "synthetic (uniform [-1,1) float64, device-resident)",
         "config": {"workload": args.workload, "M": M, "N": N, "K": K,
                    "op": ["NN", "TN (MultT1)", "NT (MultT2)"][trans],
-                   "path": "mrm_dev_dgemm -> dgemm_mfma_kernel (v_mfma_f64_16x16x4_f64)"},
+                   "path": "mrm_dev_dgemm -> dgemm_mfma_kernel (v_mfma_f64_4x4x4_4b_f64)"},
         "pct_of_peak": round(100.0 * value / FP64_PEAK_TFLOPS, 2),
         "roofline": {
             "bound": "mfma",

@@ -13,7 +13,8 @@
  *     operands through device memory and return when the result is back in
  *     host memory);
  *   - device-pointer entries (mrm_dev_*) that enqueue on a caller-supplied HIP
- *     stream (passed as void*, NULL = the library's stream) and return at once.
+ *     stream (a hipStream_t passed as void*; NULL = HIP's null stream) and
+ *     return at once.
  *
  * Errors: the reference's GEMM procedures return nothing, so the GEMM entries
  * record failures in a thread-local status read with mrm_last_error() /
@@ -104,6 +105,36 @@ MRM_API void mrm_blocked_mult_t2(double* dest, int64_t destLineWidth, const doub
  * GEMM entry above; no reference counterpart (device-resident callers). */
 MRM_API int32_t mrm_dev_dgemm(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                       const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, void* stream);
+
+/* ---- LU factorisation and linear solve ------------------------------------ */
+
+/* MatrixLUDecompInPlace (reference src/LinAlgLU.pas:283-295, recursion :181-281):
+ * in-place LU with partial pivoting of the width x width matrix A; L unit lower
+ * below the diagonal, U on and above it; indx[i] = row exchanged with row i at
+ * step i.  Returns MRM_OK (leOk) or MRM_SINGULAR (leSingular). */
+MRM_API int32_t mrm_lu_decomp_inplace(double* A, int64_t LineWidthA, int64_t width, int32_t* indx);
+
+/* MatrixLUBackSubst (reference src/LinAlgLU.pas:315-385): solves LU x = P b for
+ * the one right-hand side B (element stride LineWidthB bytes) in place. */
+MRM_API int32_t mrm_lu_backsubst(const double* LUDecomp, int64_t LineWidthLU, int64_t width, const int32_t* indx,
+                                 double* B, int64_t LineWidthB);
+
+/* MatrixLinEQSolve (reference src/LinAlgLU.pas:481-592): X (width x width2) =
+ * A^-1 B, followed by NumRefinments FP64 refinement steps on every column. */
+MRM_API int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, const double* B,
+                                int64_t LineWidthB, double* X, int64_t LineWidthX, int64_t width2,
+                                int64_t NumRefinments);
+
+/* Device LU of the n x n matrix A (stride lda bytes): indx as above; perm
+ * (optional, n entries) receives the row permutation, row i of P*A being row
+ * perm[i] of A.  Synchronises `stream` to return the status.  No reference
+ * counterpart (device-resident callers). */
+MRM_API int32_t mrm_dev_lu_factor(double* A, int64_t lda, int64_t n, int32_t* indx, int32_t* perm, void* stream);
+
+/* Device solve X = (LU)^-1 P B for nrhs columns using the perm of
+ * mrm_dev_lu_factor.  B and X must not overlap.  Asynchronous. */
+MRM_API int32_t mrm_dev_lu_solve(const double* LU, int64_t lda, int64_t n, const int32_t* perm, const double* B,
+                                 int64_t ldb, double* X, int64_t ldx, int64_t nrhs, void* stream);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,11 @@ SIGNATURES = {
     "mrm_blocked_mult_t1": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _pd]),
     "mrm_blocked_mult_t2": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _pd]),
     "mrm_dev_dgemm": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _pv]),
+    "mrm_lu_decomp_inplace": (_i32, [_pd, _i64, _i64, _pv]),
+    "mrm_lu_backsubst": (_i32, [_pd, _i64, _i64, _pv, _pd, _i64]),
+    "mrm_lineq_solve": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64]),
+    "mrm_dev_lu_factor": (_i32, [_pd, _i64, _i64, _pv, _pv, _pv]),
+    "mrm_dev_lu_solve": (_i32, [_pd, _i64, _i64, _pv, _pd, _i64, _pd, _i64, _i64, _pv]),
 }
 
 _lib = None

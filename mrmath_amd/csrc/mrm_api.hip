@@ -92,7 +92,9 @@ int mrm_ensure_init() {
     return MRM_OK;
 }
 
-hipStream_t mrm_resolve_stream(void* s) { return s ? reinterpret_cast<hipStream_t>(s) : g_ctx.stream; }
+// Device entries run on the caller's stream; NULL is HIP's null stream, so work
+// the caller queued there (e.g. by torch on its default stream) is ordered first.
+hipStream_t mrm_resolve_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 // ---------------------------------------------------------------------------
 // Host-side GEMM staging.

@@ -1,4 +1,4 @@
-// mrm_gemm.hip -- FP64 GEMM on the gfx950 matrix cores (v_mfma_f64_16x16x4_f64).
+// mrm_gemm.hip -- FP64 GEMM on the gfx950 matrix cores (v_mfma_f64_4x4x4_4b_f64).
 //
 // Replaces the reference's blocked FMA product (BlockMatrixMultiplication[T1/T2],
 // src/BlockedMult.pas:350-724, and its micro-kernel FMAMatrixMultTransposed,
@@ -16,10 +16,29 @@
 // both panels (72 KiB); the next K-panel is fetched into registers while the
 // current one is consumed, so there is one barrier per K step.
 //
-// K ordering inside a step: lane l (i = l%16, g = l/16) reads the 4 doubles
-// k = 4g..4g+3 of its row with one 32-byte LDS read, and MFMA number kk
-// (0..3) consumes element kk.  A and B use the same mapping, so every product
-// A[m,k]*B[k,n] is formed exactly once per step.
+// K ordering inside a step: lane l (i = l%16, g = l/16) owns the 4 doubles
+// k = 4g..4g+3 of the K step, and MFMA round kk (0..3) consumes element kk of
+// both operands.  A and B use the same mapping, so every product A[m,k]*B[k,n]
+// is formed exactly once per step.
+//
+// Matrix instruction (template parameter MF).  On gfx950 the four-block
+// v_mfma_f64_4x4x4_4b_f64 runs at the full FP64 rate while
+// v_mfma_f64_16x16x4_f64 does not (pure-MFMA loops: ~70 vs ~48 TFLOP/s), so MF4
+// is the default.  Measured lane layouts (l = lane):
+//   16x16x4: A[l%16][l/16], B[l/16][l%16]; accumulator element v of lane l is
+//            C[l/16 + 4v][l%16].
+//   4x4x4_4b (four independent 4x4x4 blocks b): lane l = 16k + 4b + j supplies
+//            A_b[j][k] and B_b[k][j]; D_b[i][n] lands in lane 16i + 4b + n.
+//            The CBSZ/ABID/BLGP fields of FP64 MFMAs are negation modifiers on
+//            gfx950, so there is no operand broadcast: each block needs its own
+//            copy of its B quad.
+// The MF4 wave tile (64x64) is built from 16x16 sub-tiles p (rows) x q
+// (columns): the A fragment of sub-row p puts row 16p + l%16 (= 16p + 4b + j)
+// in lane l, and the B fragment for column quad s of sub-column q puts column
+// 16q + 4s + l%4 in every block; block b then yields C[16p + 4b + i][16q + 4s + n]
+// in lane 16i + 4b + n.  Four quads s cover the 16x16 sub-tile.
+
+#include <cstdlib>
 
 #include "mrm_common.h"
 
@@ -138,8 +157,10 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
     tn = in_group / gsize;
 }
 
+enum { MF16 = 0, MF4 = 1 };
+
 // TA: A stored K x M (MultT1).  TB: B stored N x K (MultT2).  Strides in elements.
-template <bool TA, bool TB, bool VEC>
+template <bool TA, bool TB, bool VEC, int MF>
 __global__ void __launch_bounds__(NTHREADS, 2)
 dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A, int64_t lda,
                   const double* __restrict__ B, int64_t ldb, double* __restrict__ C, int64_t ldc,
@@ -164,6 +185,8 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     la.init(A, lda, m0, M, K, tid);
     lb.init(B, ldb, n0, N, K, tid);
 
+    // MF16: acc[p][q] is one 16x16 sub-tile (4 doubles per lane).
+    // MF4:  acc[p][q] element s is column quad s of sub-tile (p, q).
     mrm_d4 acc[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
@@ -178,7 +201,7 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     __syncthreads();
 
     const int a_off = (wm * 64 + li) * SK + 4 * lg;
-    const int b_off = (wn * 64 + li) * SK + 4 * lg;
+    const int b_off = (wn * 64 + (MF == MF4 ? (lane & 3) : li)) * SK + 4 * lg;
 
     for (int64_t t = 0; t < nk; ++t) {
         const int cur = int(t & 1);
@@ -191,23 +214,43 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         }
         const double* As = smem + cur * 2 * PANEL;
         const double* Bs = As + PANEL;
+        if (MF == MF16) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            double2 a[4], b[4];
+            for (int h = 0; h < 2; ++h) {
+                double2 a[4], b[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = *reinterpret_cast<const double2*>(As + a_off + r * 16 * SK + 2 * h);
+                for (int r = 0; r < 4; ++r) a[r] = *reinterpret_cast<const double2*>(As + a_off + r * 16 * SK + 2 * h);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) b[c] = *reinterpret_cast<const double2*>(Bs + b_off + c * 16 * SK + 2 * h);
+                for (int c = 0; c < 4; ++c) b[c] = *reinterpret_cast<const double2*>(Bs + b_off + c * 16 * SK + 2 * h);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+                for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, acc[r][c], 0, 0, 0);
+                    for (int c = 0; c < 4; ++c)
+                        acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, acc[r][c], 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+                for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].y, acc[r][c], 0, 0, 0);
+                    for (int c = 0; c < 4; ++c)
+                        acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].y, acc[r][c], 0, 0, 0);
+            }
+        } else {
+#pragma unroll 1
+            for (int kk = 0; kk < 4; ++kk) {
+                double a[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = As[a_off + r * 16 * SK + kk];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    double b[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) b[q] = Bs[b_off + (c * 16 + 4 * q) * SK + kk];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[r][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[r], b[q], acc[r][c][q], 0, 0, 0);
+                }
+            }
         }
         if (more) {
             double* Ns = smem + (cur ^ 1) * 2 * PANEL;
@@ -217,17 +260,23 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         __syncthreads();
     }
 
-    // Epilogue: lane holds rows 4*lg..4*lg+3, column li of each 16x16 block.
+    // Epilogue.  MF16: element v of lane l is row l/16 + 4v, column l%16 of the
+    // sub-tile.  MF4: element s of lane l = 16i + 4b + n is row 4b + i, column 4s + n.
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int64_t col = n0 + wn * 64 + c * 16 + li;
-            if (col >= N) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t row = m0 + wm * 64 + r * 16 + 4 * lg + v;
-                if (row < M) {
+                int64_t row, col;
+                if (MF == MF16) {
+                    row = m0 + wm * 64 + r * 16 + lg + 4 * v;
+                    col = n0 + wn * 64 + c * 16 + li;
+                } else {
+                    row = m0 + wm * 64 + r * 16 + 4 * ((lane >> 2) & 3) + lg;
+                    col = n0 + wn * 64 + c * 16 + 4 * v + (lane & 3);
+                }
+                if (row < M && col < N) {
                     double* p = C + row * ldc + col;
                     const double x = acc[r][c][v];
                     if (op == MRM_DO_NONE) *p = x;
@@ -239,16 +288,34 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     }
 }
 
+template <bool TA, bool TB, int MF>
+void launch_mf(bool vec, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
+               int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, hipStream_t s) {
+    dim3 grid(unsigned(tiles_m) * unsigned(tiles_n)), block(NTHREADS);
+    if (vec)
+        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true, MF>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C,
+                           ldc, op, tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false, MF>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C,
+                           ldc, op, tiles_m, tiles_n);
+}
+
+// MRM_DGEMM_MFMA=16 selects the 16x16x4 instruction (for comparison runs).
+int mfma_scheme() {
+    static int scheme = [] {
+        const char* e = getenv("MRM_DGEMM_MFMA");
+        return (e && atoi(e) == 16) ? int(MF16) : int(MF4);
+    }();
+    return scheme;
+}
+
 template <bool TA, bool TB>
 void launch_variant(bool vec, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                     int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, hipStream_t s) {
-    dim3 grid(unsigned(tiles_m) * unsigned(tiles_n)), block(NTHREADS);
-    if (vec)
-        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n);
+    if (mfma_scheme() == MF16)
+        launch_mf<TA, TB, MF16>(vec, M, N, K, A, lda, B, ldb, C, ldc, op, tiles_m, tiles_n, s);
     else
-        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n);
+        launch_mf<TA, TB, MF4>(vec, M, N, K, A, lda, B, ldb, C, ldc, op, tiles_m, tiles_n, s);
 }
 
 // C (M x N) = / += / -= op(A) * op(B), K the inner dimension.  Strides in bytes.

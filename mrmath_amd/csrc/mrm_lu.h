@@ -1,0 +1,54 @@
+// mrm_lu.h -- internal interface of the LU / triangular-solve kernels.
+#pragma once
+
+#include "mrm_common.h"
+
+// Status a panel kernel reports when its grid barrier passed the deadline.
+#define MRM_LU_TIMEOUT 100
+
+namespace mrm {
+
+constexpr int LU_NB = 256;             // outer block: K of the trailing-update GEMM
+constexpr int LU_W = 32;               // panel width of the cooperative panel kernel
+constexpr int LU_TRSM_R = 32;          // rows per diagonal block of the blocked triangular solves
+constexpr int LU_MAX_PANEL_WG = 256;   // at most one panel workgroup per CU
+
+// Device workspace of the factorisation (one per library instance; users are
+// serialised by the host mutex).
+struct LuWork {
+    unsigned* counter = nullptr;  // grid-barrier arrivals
+    int* status = nullptr;        // MRM_OK / MRM_SINGULAR / MRM_LU_TIMEOUT
+    void* xch = nullptr;          // pivot exchange area
+};
+
+size_t lu_xch_bytes();
+
+// Element-stride GEMM in the arithmetic type T (double: FP64 MFMA).
+template <typename T>
+int gemm_t(int trans, int64_t M, int64_t N, int64_t K, const T* A, int64_t lda, const T* B, int64_t ldb, T* C,
+           int64_t ldc, int op, hipStream_t s);
+
+template <>
+inline int gemm_t<double>(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
+                          int64_t ldb, double* C, int64_t ldc, int op, hipStream_t s) {
+    return mrm_launch_dgemm(trans, M, N, K, A, lda * 8, B, ldb * 8, C, ldc * 8, op, s);
+}
+
+// In-place LU with partial pivoting of the n x n matrix A (element stride lda).
+// indx: reference pivot format; perm (optional): row i of P*A is row perm[i] of A.
+template <typename T>
+int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s);
+
+// X = (LU)^-1 P B for nrhs columns.
+template <typename T>
+int lu_solve(const T* LU, int64_t lda, int n, const int* perm, const T* B, int64_t ldb, T* X, int64_t ldx, int nrhs,
+             hipStream_t s);
+
+template <typename T>
+void add_inplace(T* X, int64_t ldx, const T* D, int64_t ldd, int rows, int cols, hipStream_t s);
+void zero_i32(int* p, int n, hipStream_t s);
+
+// Library-owned LU workspace (allocated on first use); caller holds mrm_host_mutex().
+int lu_workspace(LuWork** out);
+
+}  // namespace mrm

@@ -1,0 +1,225 @@
+// mrm_lu_api.hip -- C-ABI entries of the LU factorisation and linear solver.
+//
+// Host-pointer entries keep the reference's signatures and semantics
+// (MatrixLUDecompInPlace, MatrixLUBackSubst, MatrixLinEQSolve,
+// src/LinAlgLU.pas:283-592); device-pointer entries run on a caller's stream.
+
+#include <cstring>
+#include <vector>
+
+#include "mrm_common.h"
+#include "mrm_host.h"
+#include "mrm_lu.h"
+
+namespace mrm {
+
+int lu_workspace(LuWork** out) {
+    static LuWork ws;
+    static bool ready = false;
+    if (!ready) {
+        MRM_HIP_TRY(hipMalloc(&ws.counter, 256));
+        MRM_HIP_TRY(hipMalloc(&ws.status, 256));
+        MRM_HIP_TRY(hipMalloc(&ws.xch, lu_xch_bytes()));
+        ready = true;
+    }
+    *out = &ws;
+    return MRM_OK;
+}
+
+}  // namespace mrm
+
+namespace {
+
+// Factor on the device, then read the kernels' status (synchronises s).
+int factor_and_status(double* dA, int64_t lda, int n, int* dindx, int* dperm, hipStream_t s) {
+    mrm::LuWork* ws;
+    int rc = mrm::lu_workspace(&ws);
+    if (rc) return rc;
+    mrm::zero_i32(ws->status, 1, s);
+    rc = mrm::lu_factor<double>(dA, lda, n, dindx, dperm, *ws, s);
+    if (rc) return rc;
+    int st = 0;
+    MRM_HIP_TRY(hipMemcpyAsync(&st, ws->status, sizeof(int), hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    if (st == MRM_SINGULAR) return MRM_SINGULAR;
+    if (st != 0) {
+        mrm_set_error(MRM_HIP_ERROR, "lu: panel barrier timed out");
+        return MRM_HIP_ERROR;
+    }
+    return MRM_OK;
+}
+
+int check_square(int64_t n, int64_t lw) {
+    if (n < 0 || n > 0x7fffffffLL) {
+        mrm_set_error(MRM_BAD_DIMS, "Dimension error");
+        return MRM_BAD_DIMS;
+    }
+    if (lw < n * 8 || (lw & 7)) {
+        mrm_set_error(MRM_BAD_DIMS, "Dimension Error: LineWidth < width*8");
+        return MRM_BAD_DIMS;
+    }
+    return MRM_OK;
+}
+
+// Row i of P*A is row perm[i] of A, from the reference's sequential indx.
+std::vector<int> perm_from_indx(const int32_t* indx, int64_t n) {
+    std::vector<int> p(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) p[size_t(i)] = int(i);
+    for (int64_t i = 0; i < n; ++i) {
+        const int r = indx[i];
+        if (r != i) std::swap(p[size_t(i)], p[size_t(r)]);
+    }
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mrm_dev_lu_factor(double* A, int64_t lda, int64_t n, int32_t* indx, int32_t* perm, void* stream) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(n, lda))) return rc;
+    if (n == 0) return MRM_OK;
+    if (!A || !indx) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    return factor_and_status(A, lda / 8, int(n), indx, perm, mrm_resolve_stream(stream));
+}
+
+int32_t mrm_dev_lu_solve(const double* LU, int64_t lda, int64_t n, const int32_t* perm, const double* B, int64_t ldb,
+                         double* X, int64_t ldx, int64_t nrhs, void* stream) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(n, lda))) return rc;
+    if (nrhs < 0 || ldb < nrhs * 8 || ldx < nrhs * 8 || ((ldb | ldx) & 7)) {
+        mrm_set_error(MRM_BAD_DIMS, "solve: bad right-hand-side layout");
+        return MRM_BAD_DIMS;
+    }
+    if (n == 0 || nrhs == 0) return MRM_OK;
+    return mrm::lu_solve<double>(LU, lda / 8, int(n), perm, B, ldb / 8, X, ldx / 8, int(nrhs),
+                                 mrm_resolve_stream(stream));
+}
+
+// MatrixLUDecompInPlace (src/LinAlgLU.pas:283-295)
+int32_t mrm_lu_decomp_inplace(double* A, int64_t LineWidthA, int64_t width, int32_t* indx) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(width, LineWidthA))) return rc;
+    if (width == 0) return MRM_OK;
+    if (!A || !indx) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    const int64_t n = width;
+    void *dA, *dI;
+    if ((rc = mrm_staging(0, size_t(n * n) * 8, &dA))) return rc;
+    if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    rc = factor_and_status(static_cast<double*>(dA), n, int(n), static_cast<int*>(dI), nullptr, s);
+    if (rc != MRM_OK && rc != MRM_SINGULAR) return rc;
+    // the reference leaves A partially factored on leSingular too
+    MRM_HIP_TRY(hipMemcpy2DAsync(A, size_t(LineWidthA), dA, size_t(n) * 8, size_t(n) * 8, size_t(n),
+                                 hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipMemcpyAsync(indx, dI, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    return rc;
+}
+
+// MatrixLUBackSubst (src/LinAlgLU.pas:315-385): one right-hand side, stride LineWidthB.
+int32_t mrm_lu_backsubst(const double* LUDecomp, int64_t LineWidthLU, int64_t width, const int32_t* indx, double* B,
+                         int64_t LineWidthB) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(width, LineWidthLU))) return rc;
+    if (width == 0) return MRM_OK;
+    if (!LUDecomp || !indx || !B || LineWidthB < 8 || (LineWidthB & 7)) {
+        mrm_set_error(MRM_BAD_ARG, "bad argument");
+        return MRM_BAD_ARG;
+    }
+    const int64_t n = width;
+    std::vector<int> perm = perm_from_indx(indx, n);
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    void *dLU, *dP, *dB, *dX;
+    if ((rc = mrm_staging(1, size_t(n * n) * 8, &dLU))) return rc;
+    if ((rc = mrm_staging(5, size_t(n) * 4, &dP))) return rc;
+    if ((rc = mrm_staging(2, size_t(n) * 8, &dB))) return rc;
+    if ((rc = mrm_staging(3, size_t(n) * 8, &dX))) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(dLU, size_t(n) * 8, LUDecomp, size_t(LineWidthLU), size_t(n) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    MRM_HIP_TRY(hipMemcpyAsync(dP, perm.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
+    MRM_HIP_TRY(hipMemcpy2DAsync(dB, 8, B, size_t(LineWidthB), 8, size_t(n), hipMemcpyHostToDevice, s));
+    rc = mrm::lu_solve<double>(static_cast<const double*>(dLU), n, int(n), static_cast<const int*>(dP),
+                               static_cast<const double*>(dB), 1, static_cast<double*>(dX), 1, 1, s);
+    if (rc) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(B, size_t(LineWidthB), dX, 8, 8, size_t(n), hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    return MRM_OK;
+}
+
+// MatrixLinEQSolve (src/LinAlgLU.pas:481-592): X = A^-1 B for width2 right-hand
+// sides, then NumRefinments FP64 refinement steps r = A x - b, x -= LU^-1 r
+// (applied to every right-hand side; the reference's residual loop is correct
+// only for one).
+int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, const double* B, int64_t LineWidthB,
+                        double* X, int64_t LineWidthX, int64_t width2, int64_t NumRefinments) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(width, LineWidthA))) return rc;
+    if (width2 < 0 || LineWidthB < width2 * 8 || LineWidthX < width2 * 8 || ((LineWidthB | LineWidthX) & 7) ||
+        NumRefinments < 0) {
+        mrm_set_error(MRM_BAD_DIMS, "Dimension error");
+        return MRM_BAD_DIMS;
+    }
+    if (width == 0 || width2 == 0) return MRM_OK;
+    if (!A || !B || !X) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    const int64_t n = width, m = width2;
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    void *dA, *dLU, *dB, *dX, *dR, *dD, *dI, *dP;
+    if ((rc = mrm_staging(0, size_t(n * n) * 8, &dA))) return rc;
+    if ((rc = mrm_staging(1, size_t(n * n) * 8, &dLU))) return rc;
+    if ((rc = mrm_staging(2, size_t(n * m) * 8, &dB))) return rc;
+    if ((rc = mrm_staging(3, size_t(n * m) * 8, &dX))) return rc;
+    if ((rc = mrm_staging(4, size_t(n * m) * 8, &dR))) return rc;
+    if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
+    if ((rc = mrm_staging(6, size_t(n) * 4, &dP))) return rc;
+    if ((rc = mrm_staging(7, size_t(n * m) * 8, &dD))) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    MRM_HIP_TRY(hipMemcpyAsync(dLU, dA, size_t(n * n) * 8, hipMemcpyDeviceToDevice, s));
+    MRM_HIP_TRY(hipMemcpy2DAsync(dB, size_t(m) * 8, B, size_t(LineWidthB), size_t(m) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    double* a = static_cast<double*>(dA);
+    double* lu = static_cast<double*>(dLU);
+    double* b = static_cast<double*>(dB);
+    double* x = static_cast<double*>(dX);
+    double* r = static_cast<double*>(dR);
+    double* d = static_cast<double*>(dD);
+    int* perm = static_cast<int*>(dP);
+    rc = factor_and_status(lu, n, int(n), static_cast<int*>(dI), perm, s);
+    if (rc) return rc;
+    if ((rc = mrm::lu_solve<double>(lu, n, int(n), perm, b, m, x, m, int(m), s))) return rc;
+    for (int64_t it = 0; it < NumRefinments; ++it) {
+        // r = b - A x  (the negated residual of the reference), d = LU^-1 r, x += d
+        MRM_HIP_TRY(hipMemcpyAsync(r, b, size_t(n * m) * 8, hipMemcpyDeviceToDevice, s));
+        if ((rc = mrm_launch_dgemm(MRM_NN, n, m, n, a, n * 8, x, m * 8, r, m * 8, MRM_DO_SUB, s))) return rc;
+        if ((rc = mrm::lu_solve<double>(lu, n, int(n), perm, r, m, d, m, int(m), s))) return rc;
+        mrm::add_inplace<double>(x, m, d, m, int(n), int(m), s);
+    }
+    MRM_HIP_TRY(hipMemcpy2DAsync(X, size_t(LineWidthX), x, size_t(m) * 8, size_t(m) * 8, size_t(n),
+                                 hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    return MRM_OK;
+}
+
+}  // extern "C"

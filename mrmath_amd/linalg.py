@@ -1,0 +1,64 @@
+"""Host-side mirror of mrMath's LU entry points, bound to the GPU.
+
+Same names, argument meaning and result codes as src/LinAlgLU.pas:26-62
+(MatrixLUDecompInPlace, MatrixLUBackSubst, MatrixLinEQSolve); matrices are
+numpy float64 arrays in the reference's layout.  Every call goes through
+libmrmath_gpu.so.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .stubswitch import _mat
+
+leOk, leSingular = _lib.MRM_OK, _lib.MRM_SINGULAR
+
+
+class ELinEQSingularException(Exception):
+    """Raised by the object API on a singular system (src/DblMatrix.pas:1353-1356)."""
+
+
+def _status(rc: int) -> int:
+    if rc not in (leOk, leSingular):
+        _lib.check(rc)
+    return rc
+
+
+def MatrixLUDecompInPlace(A: np.ndarray, indx: np.ndarray) -> int:
+    """In-place LU with partial pivoting; indx (int32, width entries) receives the
+    pivot rows.  Returns leOk or leSingular (src/LinAlgLU.pas:283-295)."""
+    p, w, h, lw = _mat(A, "A")
+    if w != h:
+        raise ValueError("Dimension error: A must be square")
+    if indx.dtype != np.int32 or indx.size < w:
+        raise ValueError("indx must be an int32 array of width entries")
+    return _status(_lib.load().mrm_lu_decomp_inplace(p, lw, w, indx.ctypes.data))
+
+
+def MatrixLUBackSubst(LUDecomp: np.ndarray, indx: np.ndarray, B: np.ndarray) -> None:
+    """Solves LU x = P b in place for the first column of B (src/LinAlgLU.pas:315-385)."""
+    p, w, _, lw = _mat(LUDecomp, "LUDecomp")
+    pb, _, hb, lwb = _mat(B, "B")
+    if hb != w:
+        raise ValueError("Dimension error")
+    _lib.check(_lib.load().mrm_lu_backsubst(p, lw, w, indx.ctypes.data, pb, lwb))
+
+
+def MatrixLinEQSolve(A: np.ndarray, B: np.ndarray, X: np.ndarray, NumRefinments: int = 0) -> int:
+    """X = A^-1 B (width2 = B's width right-hand sides) plus NumRefinments FP64
+    refinement steps (src/LinAlgLU.pas:481-592).  Returns leOk or leSingular."""
+    pa, w, h, lwa = _mat(A, "A")
+    pb, w2, hb, lwb = _mat(B, "B")
+    px, wx, hx, lwx = _mat(X, "X")
+    if w != h or hb != w or hx != w or wx != w2:
+        raise ValueError("Dimension error")
+    return _status(_lib.load().mrm_lineq_solve(pa, lwa, w, pb, lwb, px, lwx, w2, NumRefinments))
+
+
+def SolveLinEQ(A: np.ndarray, B: np.ndarray, NumRefinments: int = 0) -> np.ndarray:
+    """TDoubleMatrix.SolveLinEQ (src/DblMatrix.pas:1343-1362): returns X, raises on singular."""
+    X = np.zeros((A.shape[0], B.shape[1]))
+    if MatrixLinEQSolve(A, B, X, NumRefinments) == leSingular:
+        raise ELinEQSingularException("Matrix is singular")
+    return X

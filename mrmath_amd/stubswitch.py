@@ -28,6 +28,8 @@ def _mat(a: np.ndarray, name: str) -> tuple[int, int, int, int]:
     """(address, width, height, LineWidth) of a row-major float64 matrix view."""
     if not isinstance(a, np.ndarray) or a.dtype != np.float64 or a.ndim != 2:
         raise TypeError(f"{name} must be a 2-D float64 numpy array")
+    if a.size == 0:
+        return a.ctypes.data, a.shape[1], a.shape[0], max(8, 8 * a.shape[1])
     if a.strides[1] != 8 or a.strides[0] < 8 * a.shape[1]:
         raise ValueError(f"{name} must have unit column stride and row stride >= width*8")
     return a.ctypes.data, a.shape[1], a.shape[0], a.strides[0]

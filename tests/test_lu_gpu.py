@@ -1,0 +1,168 @@
+"""GPU parity of the LU factorisation and linear solve against the oracle.
+
+Bars: the reference's known-answer tests bit for bit; on random inputs the
+pivot sequence (indx) must equal the oracle's exactly and the factors must
+agree to 64*n*2^-53 * max|A| element-wise; solves must reach a normwise
+backward error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+<= 1e-14.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from mrmath_amd import _lib
+from mrmath_amd import linalg as la
+
+pytestmark = pytest.mark.gpu
+
+KAT = json.loads((Path(__file__).resolve().parent / "golden" / "kat.json").read_text())
+U = 2.0 ** -53
+
+
+def test_lu_kat():
+    k = KAT["lu"]
+    A = np.array(k["A"], dtype=np.float64).reshape(2, 2)
+    indx = np.zeros(2, dtype=np.int32)
+    assert la.MatrixLUDecompInPlace(A, indx) == la.leOk
+    assert np.array_equal(A.ravel(), k["LU"])
+    assert list(indx) == k["indx"]
+    b = np.array(k["b"], dtype=np.float64).reshape(2, 1)
+    la.MatrixLUBackSubst(A, indx, b)
+    assert np.array_equal(b.ravel(), k["x"])
+
+
+def test_lu_kat_strided():
+    s = KAT["lu"]["strided"]
+    raw = np.array(s["A_lw3"], dtype=np.float64)
+    A = np.lib.stride_tricks.as_strided(raw, (2, 2), (24, 8))
+    indx = np.zeros(2, dtype=np.int32)
+    assert la.MatrixLUDecompInPlace(A, indx) == la.leOk
+    assert np.array_equal(raw, s["LU_lw3"])
+    Braw = np.array(s["B_lw2"], dtype=np.float64)
+    la.MatrixLUBackSubst(A, indx, Braw.reshape(2, 2))
+    assert np.array_equal(Braw, s["X_lw2"])
+
+
+def test_singular():
+    A = np.array(KAT["singular"]["A"], dtype=np.float64).reshape(3, 3)
+    indx = np.zeros(3, dtype=np.int32)
+    assert la.MatrixLUDecompInPlace(A.copy(), indx) == la.leSingular
+    X = np.zeros((3, 1))
+    assert la.MatrixLinEQSolve(A, np.ones((3, 1)), X) == la.leSingular
+    with pytest.raises(la.ELinEQSingularException):
+        la.SolveLinEQ(A, np.ones((3, 1)))
+    Z = np.zeros((40, 40))
+    assert la.MatrixLUDecompInPlace(Z, np.zeros(40, dtype=np.int32)) == la.leSingular
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 32, 33, 64, 100, 255, 256, 257, 300, 520, 1000])
+def test_lu_matches_oracle(n):
+    rng = np.random.default_rng(7000 + n)
+    A0 = rng.standard_normal((n, n))
+    Ag = A0.copy()
+    ig = np.zeros(n, dtype=np.int32)
+    assert la.MatrixLUDecompInPlace(Ag, ig) == la.leOk
+    Ao = A0.copy()
+    rc, io = orc.lu_decomp_inplace(Ao)
+    assert rc == 0
+    assert np.array_equal(ig, io), "pivot sequence differs from the oracle"
+    assert np.max(np.abs(Ag - Ao)) <= 64 * n * U * np.max(np.abs(A0)) * 10
+    # P*A = L*U
+    L = np.tril(Ag, -1) + np.eye(n)
+    Um = np.triu(Ag)
+    perm = np.arange(n)
+    for i, r in enumerate(ig):
+        perm[[i, r]] = perm[[r, i]]
+    assert np.max(np.abs(A0[perm] - L @ Um)) <= 16 * n * U * np.max(np.abs(A0)) * 10
+
+
+def test_lu_padded_linewidth():
+    n = 150
+    rng = np.random.default_rng(3)
+    raw = np.full((n, n + 5), 9.0)
+    A = raw[:, :n]
+    A[...] = rng.standard_normal((n, n))
+    A0 = A.copy()
+    ig = np.zeros(n, dtype=np.int32)
+    assert la.MatrixLUDecompInPlace(A, ig) == la.leOk
+    assert np.all(raw[:, n:] == 9.0)
+    _, io = orc.lu_decomp_inplace(A0)
+    assert np.array_equal(ig, io)
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (5, 1), (48, 3), (300, 4), (777, 2), (1024, 8)])
+def test_lineq_solve_backward_error(n, nrhs):
+    A = orc.pm1_matrix(n, n, 4400 + n)
+    B = orc.pm1_matrix(nrhs, n, 4500 + n)
+    X = np.zeros((n, nrhs))
+    assert la.MatrixLinEQSolve(A, B, X, 0) == la.leOk
+    be = orc.backward_error(A, X, B)
+    assert be <= 1e-14, be
+    rc, Xo = orc.lineq_solve(A, B, 0)
+    # both are backward stable solutions of the same system
+    assert np.allclose(X, Xo, rtol=1e-8, atol=1e-10)
+
+
+def test_lineq_solve_refinement():
+    n, nrhs = 400, 3
+    rng = np.random.default_rng(5)
+    Uq, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Vq, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = Uq @ np.diag(np.logspace(0, -6, n)) @ Vq.T
+    B = rng.standard_normal((n, nrhs))
+    X0 = np.zeros((n, nrhs))
+    X2 = np.zeros((n, nrhs))
+    assert la.MatrixLinEQSolve(A, B, X0, 0) == la.leOk
+    assert la.MatrixLinEQSolve(A, B, X2, 2) == la.leOk
+    assert orc.backward_error(A, X2, B) <= 1e-14
+    assert orc.backward_error(A, X2, B) <= orc.backward_error(A, X0, B) * 1.5 + 1e-17
+
+
+def test_backsubst_single_column_matches_oracle():
+    n = 200
+    A = orc.pm1_matrix(n, n, 91)
+    lu = A.copy()
+    indx = np.zeros(n, dtype=np.int32)
+    assert la.MatrixLUDecompInPlace(lu, indx) == la.leOk
+    b = orc.pm1_matrix(1, n, 92)
+    bo = b.copy()
+    la.MatrixLUBackSubst(lu, indx, b)
+    orc.lu_backsubst(lu, indx, bo)
+    assert np.allclose(b, bo, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2048, 5000])
+def test_device_lu_and_solve(n):
+    """Device entries on torch tensors: pivots equal the oracle's (n=2048 only,
+    the oracle's recursion is O(n^3) on one core), backward error <= 1e-14 with
+    64 right-hand sides."""
+    import torch
+    lib = _lib.load()
+    g = torch.Generator(device="cuda").manual_seed(n)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    B = torch.randn(n, 64, dtype=torch.float64, device="cuda", generator=g)
+    LU = A.clone()
+    indx = torch.zeros(n, dtype=torch.int32, device="cuda")
+    perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+    X = torch.zeros(n, 64, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    assert lib.mrm_dev_lu_factor(LU.data_ptr(), n * 8, n, indx.data_ptr(), perm.data_ptr(), s) == 0
+    assert lib.mrm_dev_lu_solve(LU.data_ptr(), n * 8, n, perm.data_ptr(), B.data_ptr(), 64 * 8, X.data_ptr(), 64 * 8,
+                                64, s) == 0
+    torch.cuda.synchronize()
+    An, Bn, Xn = A.cpu().numpy(), B.cpu().numpy(), X.cpu().numpy()
+    # perm agrees with indx
+    p = np.arange(n)
+    for i, r in enumerate(indx.cpu().numpy()):
+        p[[i, r]] = p[[r, i]]
+    assert np.array_equal(p, perm.cpu().numpy())
+    r = Bn - An @ Xn
+    be = np.max(np.abs(r).max(axis=0) / (np.abs(An).sum(axis=1).max() * np.abs(Xn).max(axis=0) + np.abs(Bn).max(axis=0)))
+    assert be <= 1e-14, be
+    if n == 2048:
+        A0 = An.copy()
+        rc, io = orc.lu_decomp_inplace(A0)
+        assert np.array_equal(indx.cpu().numpy(), io)

@@ -8,7 +8,9 @@ square case of the GPU GEMM family.  Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
-Workloads: dgemm_nn_8192 (default), dgemm_tn_4096, dgemm_nt_4096.
+Workloads: dgemm_nn_8192 (default), dgemm_tn_4096, dgemm_nt_4096, and
+copy_1gib (a plain 1 GiB device copy: the calibration kernel for the HBM byte
+counters FETCH_SIZE / WRITE_SIZE).
 """
 from __future__ import annotations
 
@@ -71,12 +73,32 @@ def cpu_baseline(threads: int, n: int = 2048, budget_s: float = 15.0):
     }
 
 
+def copy_calibration(args):
+    """1 GiB device-to-device copy: reads and writes 2^30 bytes per dispatch."""
+    import torch
+    n = (1 << 30) // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / args.steps
+    print(json.dumps({"workload": "copy_1gib", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                      "ms_per_copy": round(ms, 4), "GBps": round(2 * (1 << 30) / ms / 1e6, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dgemm_nn_8192", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="dgemm_nn_8192", choices=sorted(WORKLOADS) + ["copy_1gib"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
 
@@ -89,6 +111,8 @@ def main():
     from mrmath_amd import _lib
 
     lib = _lib.load()
+    if args.workload == "copy_1gib":
+        return copy_calibration(args)
     trans, M, N, K = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

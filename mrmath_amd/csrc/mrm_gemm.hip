@@ -4,41 +4,41 @@
 // src/BlockedMult.pas:350-724, and its micro-kernel FMAMatrixMultTransposed,
 // src/FMAMatrixOperations.pas:55-92).  One kernel family covers NN, TN (MultT1)
 // and NT (MultT2): the operand layout only changes how a 128x16 panel is read
-// from HBM; inside LDS both panels are always stored K-contiguous, so the
-// tile transposes the reference performs per K-block (src/BlockedMult.pas:435,
-// 560-561) disappear into the load pattern.  The destination operation
-// (doNone / doAdd / doSub, src/BlockedMult.pas:459-463) is fused into the
-// epilogue.
+// from HBM and laid out in LDS, so the tile transposes the reference performs
+// per K-block (src/BlockedMult.pas:435, 560-561) disappear.  The destination
+// operation (doNone / doAdd / doSub, src/BlockedMult.pas:459-463) is fused
+// into the epilogue.
 //
 // Workgroup: 256 threads = 4 waves in a 2x2 arrangement; workgroup tile
-// BM x BN = 128 x 128, K step BK = 16; each wave owns a 64x64 block of C held
-// as 4x4 MFMA accumulators (16 x 4 doubles per lane).  LDS holds two stages of
-// both panels (72 KiB); the next K-panel is fetched into registers while the
-// current one is consumed, so there is one barrier per K step.
+// BM x BN = 128 x 128, K step BK = 16; each wave owns a 64x64 block of C.  LDS
+// holds two stages of both panels (66 KiB, two workgroups per CU); the next
+// K-panel is fetched into registers while the current one is consumed, so
+// there is one barrier per K step.  Every panel is stored [k][row] in LDS
+// (row stride SR), whatever its layout in HBM, so one fragment-read path
+// serves NN, TN and NT.
 //
-// K ordering inside a step: lane l (i = l%16, g = l/16) owns the 4 doubles
-// k = 4g..4g+3 of the K step, and MFMA round kk (0..3) consumes element kk of
-// both operands.  A and B use the same mapping, so every product A[m,k]*B[k,n]
-// is formed exactly once per step.
+// Matrix instruction.  On gfx950 the four-block v_mfma_f64_4x4x4_4b_f64 runs
+// at the full FP64 rate (pure-MFMA loop: 77.6 of 78.6 TFLOP/s) while
+// v_mfma_f64_16x16x4_f64 tops out near 60-66.  Measured lane layout: lane
+// l = 16k + 4b + j supplies A_b[j][k] and B_b[k][j] of block b, and
+// D_b[i][n] lands in lane 16i + 4b + n.  The CBSZ/ABID/BLGP fields of FP64
+// MFMAs are negation modifiers on gfx950, so there is no operand broadcast:
+// every block carries its own B quad.
+// The wave tile is 4x4 sub-tiles (p, c) of 16x16.  The A fragment of sub-row
+// p puts row 16p + l%16 (= 16p + 4b + j) in lane l; the B fragment for column
+// quad q of sub-column c puts column 16c + 4q + l%4 in every block; block b
+// then yields C[16p + 4b + i][16c + 4q + n] in lane 16i + 4b + n, and four
+// quads cover the sub-tile.  The accumulator acc[p][c] element q therefore
+// holds row 16p + 4((l/4)%4) + l/16, column 16c + 4q + l%4.
 //
-// Matrix instruction (template parameter MF).  On gfx950 the four-block
-// v_mfma_f64_4x4x4_4b_f64 runs at the full FP64 rate while
-// v_mfma_f64_16x16x4_f64 does not (pure-MFMA loops: ~70 vs ~48 TFLOP/s), so MF4
-// is the default.  Measured lane layouts (l = lane):
-//   16x16x4: A[l%16][l/16], B[l/16][l%16]; accumulator element v of lane l is
-//            C[l/16 + 4v][l%16].
-//   4x4x4_4b (four independent 4x4x4 blocks b): lane l = 16k + 4b + j supplies
-//            A_b[j][k] and B_b[k][j]; D_b[i][n] lands in lane 16i + 4b + n.
-//            The CBSZ/ABID/BLGP fields of FP64 MFMAs are negation modifiers on
-//            gfx950, so there is no operand broadcast: each block needs its own
-//            copy of its B quad.
-// The MF4 wave tile (64x64) is built from 16x16 sub-tiles p (rows) x q
-// (columns): the A fragment of sub-row p puts row 16p + l%16 (= 16p + 4b + j)
-// in lane l, and the B fragment for column quad s of sub-column q puts column
-// 16q + 4s + l%4 in every block; block b then yields C[16p + 4b + i][16q + 4s + n]
-// in lane 16i + 4b + n.  Four quads s cover the 16x16 sub-tile.
-
-#include <cstdlib>
+// K ordering inside a step: the 16 k of a step are visited as 4 rounds kk of
+// 4, lane group g = l/16 taking k = 4g + kk in round kk, for A and B alike, so
+// every product A[m,k]*B[k,n] is formed exactly once.
+//
+// Pipelining: one round kk and one sub-column c form a group of 16 MFMAs
+// (256 cycles); the fragments of the next group are read from LDS while the
+// current group issues, so the matrix pipe does not wait on LDS latency
+// inside a K step.
 
 #include "mrm_common.h"
 
@@ -48,11 +48,13 @@ constexpr int BM = 128;
 constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int NTHREADS = 256;
-constexpr int SK = BK + 2;               // LDS row stride in doubles (144 B): spreads rows over banks
-constexpr int PANEL = BM * SK;           // doubles per panel stage (BM == BN)
+constexpr int SR = BM + 4;     // LDS panels are [k][row]; row stride in doubles (1056 B)
+constexpr int PANEL = BK * SR; // doubles per panel stage (BM == BN)
 
 // Panel of 128 rows ("r", the M or N index) x 16 k.  KC: element (r,k) lives at
-// base[r*ld + k] (K-contiguous rows); RC: at base[k*ld + r].  Each thread moves
+// base[r*ld + k] (K-contiguous rows); RC: at base[k*ld + r].  In LDS both are
+// stored [k][r]: the RC store is a straight 16-byte copy, the KC store
+// transposes each 2-vector.  Each thread moves
 // four 2-double vectors per K step.  The per-thread pointer is computed once;
 // the four vectors sit at uniform offsets from it and every K step advances it
 // by a uniform amount, so the loop carries one 64-bit address per panel.
@@ -63,18 +65,22 @@ struct PanelLoader {
     const double* p;     // this thread's element for j = 0 at the current K step
     int64_t jstep;       // element offset between vector j and j+1
     int64_t kstep;       // element offset of one K step
-    int64_t kpos;        // K index of this thread's first element (KC: k pair; RC: k row of j=0)
+    int64_t k0;          // K index of the current step (uniform)
     int64_t K;
+    int koff;            // KC: this thread's k pair within the step; RC: its k row (j = 0)
+    bool rows_full;      // all 128 panel rows are in range (uniform)
     unsigned rmask;      // KC: bit j = row of vector j in range; RC: bit0/bit1 = row pair in range
     double2 v[4];
 
     __device__ __forceinline__ void init(const double* __restrict__ base, int64_t ld, int64_t r0, int64_t R,
                                          int64_t K_, int tid) {
         K = K_;
+        k0 = 0;
+        rows_full = r0 + BM <= R;
         if (KC) {
             const int64_t rr = r0 + (tid >> 3);
-            kpos = (tid & 7) * 2;
-            p = base + rr * ld + kpos;
+            koff = (tid & 7) * 2;
+            p = base + rr * ld + koff;
             jstep = 32 * ld;
             kstep = BK;
             rmask = 0;
@@ -82,8 +88,8 @@ struct PanelLoader {
             for (int j = 0; j < 4; ++j) rmask |= (rr + 32 * j < R) ? (1u << j) : 0u;
         } else {
             const int64_t rc = r0 + (tid & 63) * 2;
-            kpos = tid >> 6;
-            p = base + kpos * ld + rc;
+            koff = tid >> 6;
+            p = base + int64_t(koff) * ld + rc;
             jstep = 4 * ld;
             kstep = int64_t(BK) * ld;
             rmask = (rc < R ? 1u : 0u) | (rc + 1 < R ? 2u : 0u);
@@ -91,48 +97,50 @@ struct PanelLoader {
     }
 
     __device__ __forceinline__ void load() {
-        const bool kfull = kpos + BK <= K;   // every K index of this step is in range
+        if (VEC && rows_full && k0 + BK <= K) {
+            // interior tile, full K step: unconditional 16-byte loads
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = *reinterpret_cast<const double2*>(p + j * jstep);
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const double* q = p + j * jstep;
             bool ok0, ok1;
             if (KC) {
                 const bool rok = (rmask >> j) & 1u;
-                ok0 = rok && (kfull || kpos < K);
-                ok1 = rok && (kfull || kpos + 1 < K);
+                const int64_t k = k0 + koff;
+                ok0 = rok && (k < K);
+                ok1 = rok && (k + 1 < K);
             } else {
-                const bool kok = kfull || (kpos + 4 * j < K);
+                const bool kok = k0 + koff + 4 * j < K;
                 ok0 = kok && (rmask & 1u);
                 ok1 = kok && (rmask & 2u);
             }
-            if (VEC && ok0 && ok1) {
-                v[j] = *reinterpret_cast<const double2*>(q);
-            } else {
-                double2 t;
-                t.x = ok0 ? q[0] : 0.0;
-                t.y = ok1 ? q[1] : 0.0;
-                v[j] = t;
-            }
+            double2 t;
+            t.x = ok0 ? q[0] : 0.0;
+            t.y = ok1 ? q[1] : 0.0;
+            v[j] = t;
         }
     }
 
     __device__ __forceinline__ void advance() {
         p += kstep;
-        kpos += BK;
+        k0 += BK;
     }
 
     __device__ __forceinline__ void store(double* __restrict__ P, int tid) const {
         if (KC) {
             const int r = tid >> 3, k = (tid & 7) * 2;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) *reinterpret_cast<double2*>(P + (r + 32 * j) * SK + k) = v[j];
+            for (int j = 0; j < 4; ++j) {
+                P[k * SR + r + 32 * j] = v[j].x;
+                P[(k + 1) * SR + r + 32 * j] = v[j].y;
+            }
         } else {
             const int k = tid >> 6, r = (tid & 63) * 2;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                P[r * SK + k + 4 * j] = v[j].x;
-                P[(r + 1) * SK + k + 4 * j] = v[j].y;
-            }
+            for (int j = 0; j < 4; ++j) *reinterpret_cast<double2*>(P + (k + 4 * j) * SR + r) = v[j];
         }
     }
 };
@@ -157,15 +165,21 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
     tn = in_group / gsize;
 }
 
-enum { MF16 = 0, MF4 = 1 };
+// Element (row, k) of a panel in LDS (every panel is stored [k][row]).
+template <bool KC>
+__device__ __forceinline__ int panel_index(int row, int k) {
+    return k * SR + row;
+}
 
 // TA: A stored K x M (MultT1).  TB: B stored N x K (MultT2).  Strides in elements.
-template <bool TA, bool TB, bool VEC, int MF>
+template <bool TA, bool TB, bool VEC>
 __global__ void __launch_bounds__(NTHREADS, 2)
 dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A, int64_t lda,
                   const double* __restrict__ B, int64_t ldb, double* __restrict__ C, int64_t ldc,
                   int op, int tiles_m, int tiles_n) {
     __shared__ __attribute__((aligned(16))) double smem[2 * 2 * PANEL];
+    constexpr bool AKC = !TA;   // A panel rows are M: K-contiguous unless A is stored transposed
+    constexpr bool BKC = TB;    // B panel rows are N: K-contiguous only when B is stored transposed
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -178,15 +192,11 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     const int64_t m0 = int64_t(tm) * BM;
     const int64_t n0 = int64_t(tn) * BN;
 
-    // A panel rows are M: K-contiguous unless A is stored transposed.
-    PanelLoader<!TA, VEC> la;
-    // B panel rows are N: K-contiguous only when B is stored transposed.
-    PanelLoader<TB, VEC> lb;
+    PanelLoader<AKC, VEC> la;
+    PanelLoader<BKC, VEC> lb;
     la.init(A, lda, m0, M, K, tid);
     lb.init(B, ldb, n0, N, K, tid);
 
-    // MF16: acc[p][q] is one 16x16 sub-tile (4 doubles per lane).
-    // MF4:  acc[p][q] element s is column quad s of sub-tile (p, q).
     mrm_d4 acc[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
@@ -200,8 +210,10 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     lb.store(smem + PANEL, tid);
     __syncthreads();
 
-    const int a_off = (wm * 64 + li) * SK + 4 * lg;
-    const int b_off = (wn * 64 + (MF == MF4 ? (lane & 3) : li)) * SK + 4 * lg;
+    // fragment rows of this lane: A row wm*64 + 16p + li, B column wn*64 + 16c + 4q + (lane & 3)
+    const int arow = wm * 64 + li;
+    const int bcol = wn * 64 + (lane & 3);
+    const int kq = 4 * lg;
 
     for (int64_t t = 0; t < nk; ++t) {
         const int cur = int(t & 1);
@@ -214,42 +226,33 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         }
         const double* As = smem + cur * 2 * PANEL;
         const double* Bs = As + PANEL;
-        if (MF == MF16) {
+        double a_cur[4], a_nxt[4], b_cur[4], b_nxt[4];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                double2 a[4], b[4];
+        for (int p = 0; p < 4; ++p) a_cur[p] = As[panel_index<AKC>(arow + 16 * p, kq)];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) a[r] = *reinterpret_cast<const double2*>(As + a_off + r * 16 * SK + 2 * h);
+        for (int q = 0; q < 4; ++q) b_cur[q] = Bs[panel_index<BKC>(bcol + 4 * q, kq)];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) b[c] = *reinterpret_cast<const double2*>(Bs + b_off + c * 16 * SK + 2 * h);
+        for (int gidx = 0; gidx < 16; ++gidx) {
+            const int c = gidx & 3;
+            if (gidx + 1 < 16) {
+                const int kk2 = (gidx + 1) >> 2, c2 = (gidx + 1) & 3;
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
+                for (int q = 0; q < 4; ++q) b_nxt[q] = Bs[panel_index<BKC>(bcol + 16 * c2 + 4 * q, kq + kk2)];
+                if (c2 == 0) {
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].x, b[c].x, acc[r][c], 0, 0, 0);
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r].y, b[c].y, acc[r][c], 0, 0, 0);
-            }
-        } else {
-#pragma unroll 1
-            for (int kk = 0; kk < 4; ++kk) {
-                double a[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) a[r] = As[a_off + r * 16 * SK + kk];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    double b[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) b[q] = Bs[b_off + (c * 16 + 4 * q) * SK + kk];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            acc[r][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[r], b[q], acc[r][c][q], 0, 0, 0);
+                    for (int p = 0; p < 4; ++p) a_nxt[p] = As[panel_index<AKC>(arow + 16 * p, kq + kk2)];
                 }
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc[p][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a_cur[p], b_cur[q], acc[p][c][q], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b_cur[q] = b_nxt[q];
+            if (((gidx + 1) & 3) == 0) {
+#pragma unroll
+                for (int p = 0; p < 4; ++p) a_cur[p] = a_nxt[p];
             }
         }
         if (more) {
@@ -260,62 +263,41 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         __syncthreads();
     }
 
-    // Epilogue.  MF16: element v of lane l is row l/16 + 4v, column l%16 of the
-    // sub-tile.  MF4: element s of lane l = 16i + 4b + n is row 4b + i, column 4s + n.
+    // Epilogue: acc[p][c] element q of lane l is row 16p + 4((l/4)%4) + l/16,
+    // column 16c + 4q + l%4 of the wave tile.
+    const int erow = 4 * ((lane >> 2) & 3) + lg;
+    const int ecol = lane & 3;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int p = 0; p < 4; ++p) {
+        const int64_t row = m0 + wm * 64 + 16 * p + erow;
+        if (row >= M) continue;
+        double* crow = C + row * ldc;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int64_t row, col;
-                if (MF == MF16) {
-                    row = m0 + wm * 64 + r * 16 + lg + 4 * v;
-                    col = n0 + wn * 64 + c * 16 + li;
-                } else {
-                    row = m0 + wm * 64 + r * 16 + 4 * ((lane >> 2) & 3) + lg;
-                    col = n0 + wn * 64 + c * 16 + 4 * v + (lane & 3);
-                }
-                if (row < M && col < N) {
-                    double* p = C + row * ldc + col;
-                    const double x = acc[r][c][v];
-                    if (op == MRM_DO_NONE) *p = x;
-                    else if (op == MRM_DO_ADD) *p = *p + x;
-                    else *p = *p - x;
+            for (int q = 0; q < 4; ++q) {
+                const int64_t col = n0 + wn * 64 + 16 * c + 4 * q + ecol;
+                if (col < N) {
+                    const double x = acc[p][c][q];
+                    if (op == MRM_DO_NONE) crow[col] = x;
+                    else if (op == MRM_DO_ADD) crow[col] = crow[col] + x;
+                    else crow[col] = crow[col] - x;
                 }
             }
         }
     }
 }
 
-template <bool TA, bool TB, int MF>
-void launch_mf(bool vec, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
-               int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, hipStream_t s) {
-    dim3 grid(unsigned(tiles_m) * unsigned(tiles_n)), block(NTHREADS);
-    if (vec)
-        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true, MF>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C,
-                           ldc, op, tiles_m, tiles_n);
-    else
-        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false, MF>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C,
-                           ldc, op, tiles_m, tiles_n);
-}
-
-// MRM_DGEMM_MFMA=16 selects the 16x16x4 instruction (for comparison runs).
-int mfma_scheme() {
-    static int scheme = [] {
-        const char* e = getenv("MRM_DGEMM_MFMA");
-        return (e && atoi(e) == 16) ? int(MF16) : int(MF4);
-    }();
-    return scheme;
-}
-
 template <bool TA, bool TB>
 void launch_variant(bool vec, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                     int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, hipStream_t s) {
-    if (mfma_scheme() == MF16)
-        launch_mf<TA, TB, MF16>(vec, M, N, K, A, lda, B, ldb, C, ldc, op, tiles_m, tiles_n, s);
+    dim3 grid(unsigned(tiles_m) * unsigned(tiles_n)), block(NTHREADS);
+    if (vec)
+        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
+                           op, tiles_m, tiles_n);
     else
-        launch_mf<TA, TB, MF4>(vec, M, N, K, A, lda, B, ldb, C, ldc, op, tiles_m, tiles_n, s);
+        hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
+                           op, tiles_m, tiles_n);
 }
 
 // C (M x N) = / += / -= op(A) * op(B), K the inner dimension.  Strides in bytes.

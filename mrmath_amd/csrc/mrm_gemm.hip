@@ -3,19 +3,28 @@
 // Replaces the reference's blocked FMA product (BlockMatrixMultiplication[T1/T2],
 // src/BlockedMult.pas:350-724, and its micro-kernel FMAMatrixMultTransposed,
 // src/FMAMatrixOperations.pas:55-92).  One kernel family covers NN, TN (MultT1)
-// and NT (MultT2): the operand layout only changes how a 128x16 panel is read
-// from HBM and laid out in LDS, so the tile transposes the reference performs
-// per K-block (src/BlockedMult.pas:435, 560-561) disappear.  The destination
-// operation (doNone / doAdd / doSub, src/BlockedMult.pas:459-463) is fused
-// into the epilogue.
+// and NT (MultT2): the operand layout only changes how a 128x16 panel is
+// moved from HBM into LDS and addressed there, so the tile transposes the
+// reference performs per K-block (src/BlockedMult.pas:435, 560-561) disappear.
+// The destination operation (doNone / doAdd / doSub, src/BlockedMult.pas:459-463)
+// is fused into the epilogue.
 //
 // Workgroup: 256 threads = 4 waves in a 2x2 arrangement; workgroup tile
 // BM x BN = 128 x 128, K step BK = 16; each wave owns a 64x64 block of C.  LDS
-// holds two stages of both panels (66 KiB, two workgroups per CU); the next
-// K-panel is fetched into registers while the current one is consumed, so
-// there is one barrier per K step.  Every panel is stored [k][row] in LDS
-// (row stride SR), whatever its layout in HBM, so one fragment-read path
-// serves NN, TN and NT.
+// holds two stages of both panels (two workgroups per CU).  Interior K steps
+// move each panel straight from HBM into LDS with global_load_lds_dwordx4
+// (no register round trip, no LDS store instructions); the load of step t+1
+// is issued before step t is consumed and one barrier per step publishes it.
+// Edge tiles and partial K steps take a predicated register path, after the
+// step's compute, that writes the same LDS layout.
+//
+// LDS panel layouts (r = the panel's M or N index):
+//   KC (operand K-contiguous in HBM): [r][k], 16 doubles per row, the eight
+//       16-byte chunks of row r stored at slot (chunk ^ ((r/2) % 8)) so that
+//       the fragment reads of 16 consecutive rows hit distinct banks;
+//   RC (operand r-contiguous in HBM): [k][r], row stride SR = 132 doubles.
+// One direct-load instruction of a wave writes 1 KiB of LDS (64 lanes x 16 B):
+// eight KC rows, or one RC k-row.
 //
 // Matrix instruction.  On gfx950 the four-block v_mfma_f64_4x4x4_4b_f64 runs
 // at the full FP64 rate (pure-MFMA loop: 77.6 of 78.6 TFLOP/s) while
@@ -23,7 +32,9 @@
 // l = 16k + 4b + j supplies A_b[j][k] and B_b[k][j] of block b, and
 // D_b[i][n] lands in lane 16i + 4b + n.  The CBSZ/ABID/BLGP fields of FP64
 // MFMAs are negation modifiers on gfx950, so there is no operand broadcast:
-// every block carries its own B quad.
+// every block carries its own B quad.  (Deriving the quads with DPP row
+// rotations instead of LDS reads was measured slower: FP64 VALU work competes
+// with FP64 MFMA issue.)
 // The wave tile is 4x4 sub-tiles (p, c) of 16x16.  The A fragment of sub-row
 // p puts row 16p + l%16 (= 16p + 4b + j) in lane l; the B fragment for column
 // quad q of sub-column c puts column 16c + 4q + l%4 in every block; block b
@@ -32,13 +43,14 @@
 // holds row 16p + 4((l/4)%4) + l/16, column 16c + 4q + l%4.
 //
 // K ordering inside a step: the 16 k of a step are visited as 4 rounds kk of
-// 4, lane group g = l/16 taking k = 4g + kk in round kk, for A and B alike, so
-// every product A[m,k]*B[k,n] is formed exactly once.
+// 4, lane group g = l/16 taking k = g + 4kk in round kk, for A and B alike, so
+// every product A[m,k]*B[k,n] is formed exactly once (and no two rounds read
+// neighbouring doubles of a KC row, which would invite the compiler to merge
+// the reads and hold them longer).
 //
 // Pipelining: one round kk and one sub-column c form a group of 16 MFMAs
 // (256 cycles); the fragments of the next group are read from LDS while the
-// current group issues, so the matrix pipe does not wait on LDS latency
-// inside a K step.
+// current group issues.
 
 #include "mrm_common.h"
 
@@ -48,102 +60,122 @@ constexpr int BM = 128;
 constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int NTHREADS = 256;
-constexpr int SR = BM + 4;     // LDS panels are [k][row]; row stride in doubles (1056 B)
-constexpr int PANEL = BK * SR; // doubles per panel stage (BM == BN)
+constexpr int SR = BM + 4;                          // RC panels: [k][r] row stride in doubles
+constexpr int KC_PANEL = BM * BK;                   // KC panels: 128 rows x 16 doubles
+constexpr int RC_PANEL = BK * SR;
+constexpr int PANEL = KC_PANEL > RC_PANEL ? KC_PANEL : RC_PANEL;   // doubles per panel stage
 
-// Panel of 128 rows ("r", the M or N index) x 16 k.  KC: element (r,k) lives at
-// base[r*ld + k] (K-contiguous rows); RC: at base[k*ld + r].  In LDS both are
-// stored [k][r]: the RC store is a straight 16-byte copy, the KC store
-// transposes each 2-vector.  Each thread moves
-// four 2-double vectors per K step.  The per-thread pointer is computed once;
-// the four vectors sit at uniform offsets from it and every K step advances it
-// by a uniform amount, so the loop carries one 64-bit address per panel.
-//   KC: thread -> row rr = tid/8 (+32j), k pair kc = 2*(tid%8)
-//   RC: thread -> k  kr = tid/64 (+4j),  row pair rc = 2*(tid%64)
+typedef const __attribute__((address_space(1))) void* gptr_t;
+typedef __attribute__((address_space(3))) void* lptr_t;
+
+__device__ __forceinline__ int kc_swizzle(int r) { return (r >> 1) & 7; }
+
+// LDS element index of (r, k) in a panel.
+template <bool KC>
+__device__ __forceinline__ int panel_index(int r, int k) {
+    if (KC) return r * BK + 2 * ((k >> 1) ^ kc_swizzle(r)) + (k & 1);
+    return k * SR + r;
+}
+
+// Moves one 128 x 16 panel per K step.  Wave w, instruction i, lane L:
+//   KC: panel row r = 32w + 8i + L/8, LDS slot L%8 <- HBM chunk (L%8) ^ swz(r);
+//       swz(r + 8) = swz(r) ^ 4, so odd instructions read chunk offset ^ 4
+//   RC: panel k-row 4w + i, rows r = 2L, 2L+1
+// base points at the operand's element (0, 0); ld in elements.
 template <bool KC, bool VEC>
-struct PanelLoader {
-    const double* p;     // this thread's element for j = 0 at the current K step
-    int64_t jstep;       // element offset between vector j and j+1
+struct PanelMover {
+    const double* p;     // this lane's source for instruction 0 at the current step
+    int64_t istep;       // element offset between instructions i and i+1
     int64_t kstep;       // element offset of one K step
     int64_t k0;          // K index of the current step (uniform)
     int64_t K;
-    int koff;            // KC: this thread's k pair within the step; RC: its k row (j = 0)
-    bool rows_full;      // all 128 panel rows are in range (uniform)
-    unsigned rmask;      // KC: bit j = row of vector j in range; RC: bit0/bit1 = row pair in range
-    double2 v[4];
+    int lds0;            // LDS element of this lane's 16 bytes, instruction 0
+    int lds_wave0;       // LDS element where this wave's instruction 0 writes (uniform)
+    int ldsstep;         // LDS elements between instructions
+    int koff;            // KC: this lane's first k (even instructions); RC: this wave's first k-row
+    int kodd;            // KC: element shift of odd instructions' k (koff ^ 8 - koff)
+    bool rows_full;      // all 128 panel rows exist (uniform)
+    unsigned rmask;      // KC: bit i = row of instruction i exists; RC: bit0/1 = row pair exists
+    double2 v[4];        // register path staging
 
     __device__ __forceinline__ void init(const double* __restrict__ base, int64_t ld, int64_t r0, int64_t R,
                                          int64_t K_, int tid) {
+        const int w = tid >> 6, L = tid & 63;
         K = K_;
         k0 = 0;
         rows_full = r0 + BM <= R;
         if (KC) {
-            const int64_t rr = r0 + (tid >> 3);
-            koff = (tid & 7) * 2;
-            p = base + rr * ld + koff;
-            jstep = 32 * ld;
+            const int r = 32 * w + (L >> 3);
+            const int slot = L & 7;
+            koff = 2 * (slot ^ kc_swizzle(r));
+            kodd = (koff ^ 8) - koff;
+            p = base + (r0 + r) * ld + koff;
+            istep = 8 * ld;
             kstep = BK;
+            lds0 = r * BK + 2 * slot;
+            lds_wave0 = 32 * w * BK;
+            ldsstep = 8 * BK;
             rmask = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) rmask |= (rr + 32 * j < R) ? (1u << j) : 0u;
+            for (int i = 0; i < 4; ++i) rmask |= (r0 + r + 8 * i < R) ? (1u << i) : 0u;
         } else {
-            const int64_t rc = r0 + (tid & 63) * 2;
-            koff = tid >> 6;
-            p = base + int64_t(koff) * ld + rc;
-            jstep = 4 * ld;
+            koff = 4 * w;
+            kodd = 0;
+            p = base + int64_t(koff) * ld + r0 + 2 * L;
+            istep = ld;
             kstep = int64_t(BK) * ld;
-            rmask = (rc < R ? 1u : 0u) | (rc + 1 < R ? 2u : 0u);
+            lds0 = koff * SR + 2 * L;
+            lds_wave0 = koff * SR;
+            ldsstep = SR;
+            rmask = (r0 + 2 * L < R ? 1u : 0u) | (r0 + 2 * L + 1 < R ? 2u : 0u);
         }
     }
 
-    __device__ __forceinline__ void load() {
-        if (VEC && rows_full && k0 + BK <= K) {
-            // interior tile, full K step: unconditional 16-byte loads
+    // Interior step of a 16-byte aligned operand: HBM -> LDS directly.
+    __device__ __forceinline__ bool direct_ok() const { return VEC && rows_full && k0 + BK <= K; }
+
+    __device__ __forceinline__ void issue_direct(double* __restrict__ P) const {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = *reinterpret_cast<const double2*>(p + j * jstep);
-            return;
-        }
+        for (int i = 0; i < 4; ++i)
+            __builtin_amdgcn_global_load_lds((gptr_t)(p + i * istep + (i & 1) * kodd),
+                                             (lptr_t)(P + lds_wave0 + i * ldsstep), 16, 0, 0);
+    }
+
+    // Predicated path: registers, then store().
+    __device__ __forceinline__ void load_regs() {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const double* q = p + j * jstep;
+        for (int i = 0; i < 4; ++i) {
+            const double* q = p + i * istep + (i & 1) * kodd;
             bool ok0, ok1;
             if (KC) {
-                const bool rok = (rmask >> j) & 1u;
-                const int64_t k = k0 + koff;
+                const bool rok = (rmask >> i) & 1u;
+                const int64_t k = k0 + koff + (i & 1) * kodd;
                 ok0 = rok && (k < K);
                 ok1 = rok && (k + 1 < K);
             } else {
-                const bool kok = k0 + koff + 4 * j < K;
+                const bool kok = k0 + koff + i < K;
                 ok0 = kok && (rmask & 1u);
                 ok1 = kok && (rmask & 2u);
             }
             double2 t;
             t.x = ok0 ? q[0] : 0.0;
             t.y = ok1 ? q[1] : 0.0;
-            v[j] = t;
+            v[i] = t;
         }
+    }
+
+    __device__ __forceinline__ void store(double* __restrict__ P) const {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) *reinterpret_cast<double2*>(P + lds0 + i * ldsstep) = v[i];
     }
 
     __device__ __forceinline__ void advance() {
         p += kstep;
         k0 += BK;
     }
-
-    __device__ __forceinline__ void store(double* __restrict__ P, int tid) const {
-        if (KC) {
-            const int r = tid >> 3, k = (tid & 7) * 2;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                P[k * SR + r + 32 * j] = v[j].x;
-                P[(k + 1) * SR + r + 32 * j] = v[j].y;
-            }
-        } else {
-            const int k = tid >> 6, r = (tid & 63) * 2;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) *reinterpret_cast<double2*>(P + (k + 4 * j) * SR + r) = v[j];
-        }
-    }
 };
+
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Workgroup id -> (tile_m, tile_n).  Workgroups are dealt round-robin to the 8
 // XCDs; remap so that each XCD receives a contiguous run of logical tiles, then
@@ -163,12 +195,6 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
     const int in_group = logical - group * per_group;
     tm = first_m + in_group % gsize;
     tn = in_group / gsize;
-}
-
-// Element (row, k) of a panel in LDS (every panel is stored [k][row]).
-template <bool KC>
-__device__ __forceinline__ int panel_index(int row, int k) {
-    return k * SR + row;
 }
 
 // TA: A stored K x M (MultT1).  TB: B stored N x K (MultT2).  Strides in elements.
@@ -192,8 +218,8 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     const int64_t m0 = int64_t(tm) * BM;
     const int64_t n0 = int64_t(tn) * BN;
 
-    PanelLoader<AKC, VEC> la;
-    PanelLoader<BKC, VEC> lb;
+    PanelMover<AKC, VEC> la;
+    PanelMover<BKC, VEC> lb;
     la.init(A, lda, m0, M, K, tid);
     lb.init(B, ldb, n0, N, K, tid);
 
@@ -204,25 +230,41 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         for (int c = 0; c < 4; ++c) acc[r][c] = mrm_d4{0.0, 0.0, 0.0, 0.0};
 
     const int64_t nk = (K + BK - 1) / BK;
-    la.load();
-    lb.load();
-    la.store(smem, tid);
-    lb.store(smem + PANEL, tid);
+    // stage 0
+    if (la.direct_ok()) {
+        la.issue_direct(smem);
+    } else {
+        la.load_regs();
+        la.store(smem);
+    }
+    if (lb.direct_ok()) {
+        lb.issue_direct(smem + PANEL);
+    } else {
+        lb.load_regs();
+        lb.store(smem + PANEL);
+    }
+    wait_vmem();
     __syncthreads();
 
     // fragment rows of this lane: A row wm*64 + 16p + li, B column wn*64 + 16c + 4q + (lane & 3)
     const int arow = wm * 64 + li;
     const int bcol = wn * 64 + (lane & 3);
-    const int kq = 4 * lg;
+    const int kq = lg;   // round kk reads k = lg + 4kk
 
     for (int64_t t = 0; t < nk; ++t) {
         const int cur = int(t & 1);
         const bool more = (t + 1) < nk;
+        double* Ns = smem + (cur ^ 1) * 2 * PANEL;
+        bool a_regs = false, b_regs = false;
         if (more) {
             la.advance();
             lb.advance();
-            la.load();
-            lb.load();
+            // the predicated path (edge tiles, partial K step) runs after the
+            // compute so its staging registers are not live across it
+            if (la.direct_ok()) la.issue_direct(Ns);
+            else a_regs = true;
+            if (lb.direct_ok()) lb.issue_direct(Ns + PANEL);
+            else b_regs = true;
         }
         const double* As = smem + cur * 2 * PANEL;
         const double* Bs = As + PANEL;
@@ -230,17 +272,17 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
 #pragma unroll
         for (int p = 0; p < 4; ++p) a_cur[p] = As[panel_index<AKC>(arow + 16 * p, kq)];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) b_cur[q] = Bs[panel_index<BKC>(bcol + 4 * q, kq)];
+        for (int q = 0; q < 4; ++q) b_cur[q] = Bs[panel_index<BKC>(bcol + 4 * q, kq)];   // round 0
 #pragma unroll
         for (int gidx = 0; gidx < 16; ++gidx) {
             const int c = gidx & 3;
             if (gidx + 1 < 16) {
                 const int kk2 = (gidx + 1) >> 2, c2 = (gidx + 1) & 3;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) b_nxt[q] = Bs[panel_index<BKC>(bcol + 16 * c2 + 4 * q, kq + kk2)];
+                for (int q = 0; q < 4; ++q) b_nxt[q] = Bs[panel_index<BKC>(bcol + 16 * c2 + 4 * q, kq + 4 * kk2)];
                 if (c2 == 0) {
 #pragma unroll
-                    for (int p = 0; p < 4; ++p) a_nxt[p] = As[panel_index<AKC>(arow + 16 * p, kq + kk2)];
+                    for (int p = 0; p < 4; ++p) a_nxt[p] = As[panel_index<AKC>(arow + 16 * p, kq + 4 * kk2)];
                 }
             }
 #pragma unroll
@@ -255,11 +297,15 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
                 for (int p = 0; p < 4; ++p) a_cur[p] = a_nxt[p];
             }
         }
-        if (more) {
-            double* Ns = smem + (cur ^ 1) * 2 * PANEL;
-            la.store(Ns, tid);
-            lb.store(Ns + PANEL, tid);
+        if (a_regs) {
+            la.load_regs();
+            la.store(Ns);
         }
+        if (b_regs) {
+            lb.load_regs();
+            lb.store(Ns + PANEL);
+        }
+        wait_vmem();
         __syncthreads();
     }
 
@@ -300,14 +346,15 @@ void launch_variant(bool vec, int64_t M, int64_t N, int64_t K, const double* A, 
                            op, tiles_m, tiles_n);
 }
 
-// C (M x N) = / += / -= op(A) * op(B), K the inner dimension.  Strides in bytes.
-__global__ void scale_zero_kernel(double* C, int64_t ldc, int64_t M, int64_t N) {
+// C (M x N) = 0: the empty product for doNone.
+__global__ void zero_matrix_kernel(double* C, int64_t ldc, int64_t M, int64_t N) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx < M * N) C[(idx / N) * ldc + idx % N] = 0.0;
 }
 
 }  // namespace
 
+// C (M x N) = / += / -= op(A) * op(B), K the inner dimension.  Strides in bytes.
 int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                      const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream) {
     if (M < 0 || N < 0 || K < 0 || op < 0 || op > 2 || trans < 0 || trans > 2) {
@@ -325,7 +372,7 @@ int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A
         // Empty product: dest = 0 for doNone, unchanged for doAdd / doSub.
         if (op == MRM_DO_NONE) {
             const int64_t n = M * N;
-            hipLaunchKernelGGL(scale_zero_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, C, ec, M,
+            hipLaunchKernelGGL(zero_matrix_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, C, ec, M,
                                N);
             MRM_HIP_TRY(hipGetLastError());
         }

@@ -10,8 +10,9 @@ namespace mrm {
 
 constexpr int LU_NB = 256;             // outer block: K of the trailing-update GEMM
 constexpr int LU_W = 32;               // panel width of the cooperative panel kernel
-constexpr int LU_TRSM_R = 32;          // rows per diagonal block of the blocked triangular solves
-constexpr int LU_MAX_PANEL_WG = 256;   // at most one panel workgroup per CU
+constexpr int LU_TRSM_R = 64;          // rows per diagonal block of the blocked triangular solves
+constexpr int LU_MAX_PANEL_WG = 128;   // panel workgroups (all resident; candidates fit in LDS)
+constexpr int LU_PANEL_WG_TARGET = 16; // preferred panel workgroups before packing more rows per thread
 
 // Device workspace of the factorisation (one per library instance; users are
 // serialised by the host mutex).

@@ -65,25 +65,27 @@ __device__ __forceinline__ void wave_argmax(T& v, int& r) {
 }
 
 // Grid-wide barrier for a cooperative launch: monotonic arrival counter.
-// Every workgroup leaves after `deadline` (wall clock, 100 MHz) at the latest
-// and then reports MRM_LU_TIMEOUT, so no wave can spin for ever.
+// One release fence before arriving and one acquire fence after the wait
+// (polling itself uses relaxed loads, so it does not invalidate caches on
+// every poll).  Every workgroup leaves after `deadline` (wall clock, 100 MHz)
+// at the latest and then reports MRM_LU_TIMEOUT, so no wave spins for ever.
 __device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned target, uint64_t deadline,
                                              int* status) {
     __syncthreads();
     __shared__ int s_ok;
     if (threadIdx.x == 0) {
         int ok = 1;
-        __threadfence();
-        atomicAdd(counter, 1u);
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (wall_clock64() > deadline) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned polls = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if ((++polls & 63u) == 0 && wall_clock64() > deadline) {
                 atomicExch(status, MRM_LU_TIMEOUT);
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
     }
     __syncthreads();
@@ -94,10 +96,12 @@ template <typename T, int W>
 struct PanelShared {
     T prow[W];                  // pivot row of the current column
     T drow[W];                  // row j before the exchange
-    T wv[PANEL_THREADS / 64];   // per-wave reduction scratch
+    T mine[W];                  // this workgroup's candidate row
+    T wv[PANEL_THREADS / 64];   // per-wave candidates
     int wr[PANEL_THREADS / 64];
-    int ww[PANEL_THREADS / 64];
-    int piv, pwg, sing;
+    T cv[mrm::LU_MAX_PANEL_WG];      // every workgroup's candidate |value|,
+    int cr[mrm::LU_MAX_PANEL_WG];    //   its row,
+    T cd[mrm::LU_MAX_PANEL_WG * W];  //   and the whole candidate row
 };
 
 // Exchange area of one column parity: [G values][G rows][G*W row data][W diag row].
@@ -117,15 +121,27 @@ struct Exchange {
     }
 };
 
+template <typename T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_agent(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One column j of the panel (j is a compile-time constant after unrolling, so
-// v stays in registers).  Returns false when the grid stops (singular column or
-// barrier timeout).
+// v stays in registers).  Critical path: local argmax, parallel publication of
+// the candidate row (and of row j by its owner), one grid barrier, one parallel
+// read of all candidates into LDS, redundant selection by every thread.
+// Returns false when the grid stops (singular column or barrier timeout).
 template <typename T, int W, int RPT>
 __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const int (&lr)[RPT], int m, int g, int G,
                                              int t, int lane, int wave, int piv_base, int* __restrict__ indx,
                                              int* __restrict__ perm, T* __restrict__ xch,
                                              unsigned* __restrict__ counter, int* __restrict__ status,
                                              uint64_t deadline, PanelShared<T, W>& sh) {
+    constexpr int NW = PANEL_THREADS / 64;
     Exchange<T, W> x(xch, G, j & 1);
     // (a) best candidate of this workgroup among rows j..m-1
     T best = T(-1);
@@ -146,95 +162,63 @@ __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const 
         sh.wr[wave] = brow;
     }
     __syncthreads();
-    if (t == 0) {
-        for (int q = 1; q < PANEL_THREADS / 64; ++q)
-            if (better(sh.wv[q], sh.wr[q], best, brow)) {
-                best = sh.wv[q];
-                brow = sh.wr[q];
-            }
-        sh.piv = brow;
-        __hip_atomic_store(&x.val[g], best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&x.row[g], brow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    const int wgrow = sh.piv;
-    // (b) publish the candidate row and, from its owner, the current row j
+    best = sh.wv[0];
+    brow = sh.wr[0];
+#pragma unroll
+    for (int q = 1; q < NW; ++q)
+        if (better(sh.wv[q], sh.wr[q], best, brow)) {
+            best = sh.wv[q];
+            brow = sh.wr[q];
+        }
+    // (b) the owners put the candidate row and row j into LDS ...
+    const bool own_j = (j >= g * RPT * PANEL_THREADS) && (j < (g + 1) * RPT * PANEL_THREADS);
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        if (lr[r] == wgrow) {
+        if (lr[r] == brow) {
 #pragma unroll
-            for (int c = 0; c < W; ++c)
-                __hip_atomic_store(&x.data[size_t(g) * W + c], v[r][c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int c = 0; c < W; ++c) sh.mine[c] = v[r][c];
         }
         if (lr[r] == j) {
 #pragma unroll
-            for (int c = 0; c < W; ++c)
-                __hip_atomic_store(&x.diag[c], v[r][c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int c = 0; c < W; ++c) sh.drow[c] = v[r][c];
         }
+    }
+    __syncthreads();
+    // ... and the workgroup publishes them with one store per thread
+    if (t < W) {
+        st_agent(&x.data[size_t(g) * W + t], sh.mine[t]);
+        if (own_j) st_agent(&x.diag[t], sh.drow[t]);
+    } else if (t == W) {
+        st_agent(&x.val[g], best);
+    } else if (t == W + 1) {
+        st_agent(&x.row[g], brow);
     }
     // (c) all workgroups agree
     if (!grid_barrier(counter, unsigned(G) * unsigned(j + 1), deadline, status)) return false;
-    // (d) global pivot: max |value|, earliest row on ties; carry the owning workgroup
-    T gb = T(-1);
-    int gr = INT_MAX, gw = 0;
+    // (d) one parallel read of every candidate (and of row j) into LDS
     for (int q = t; q < G; q += PANEL_THREADS) {
-        const T cv = __hip_atomic_load(&x.val[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int cr = __hip_atomic_load(&x.row[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (better(cv, cr, gb, gr)) {
-            gb = cv;
-            gr = cr;
+        sh.cv[q] = ld_agent(&x.val[q]);
+        sh.cr[q] = ld_agent(&x.row[q]);
+    }
+    for (int e = t; e < G * W; e += PANEL_THREADS) sh.cd[e] = ld_agent(&x.data[e]);
+    if (!own_j && t < W) sh.drow[t] = ld_agent(&x.diag[t]);
+    __syncthreads();
+    // every thread selects the pivot itself: max |value|, earliest row on ties
+    T gb = sh.cv[0];
+    int gr = sh.cr[0], gw = 0;
+    for (int q = 1; q < G; ++q)
+        if (better(sh.cv[q], sh.cr[q], gb, gr)) {
+            gb = sh.cv[q];
+            gr = sh.cr[q];
             gw = q;
         }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(gb, off, 64);
-        const int orow = __shfl_xor(gr, off, 64);
-        const int ow = __shfl_xor(gw, off, 64);
-        if (better(ov, orow, gb, gr)) {
-            gb = ov;
-            gr = orow;
-            gw = ow;
-        }
-    }
-    if (lane == 0) {
-        sh.wv[wave] = gb;
-        sh.wr[wave] = gr;
-        sh.ww[wave] = gw;
-    }
-    __syncthreads();
-    if (t == 0) {
-        T b0 = sh.wv[0];
-        int r0 = sh.wr[0], w0 = sh.ww[0];
-        for (int q = 1; q < PANEL_THREADS / 64; ++q)
-            if (better(sh.wv[q], sh.wr[q], b0, r0)) {
-                b0 = sh.wv[q];
-                r0 = sh.wr[q];
-                w0 = sh.ww[q];
-            }
-        sh.piv = r0;
-        sh.pwg = w0;
-        sh.sing = !(b0 > Lim<T>::tiny());
-    }
-    __syncthreads();
-    if (sh.sing) {
+    if (!(gb > Lim<T>::tiny())) {
         if (g == 0 && t == 0) atomicExch(status, MRM_SINGULAR);
         return false;
     }
-    const int prow = sh.piv;
-    if (t < W) {
-        sh.prow[t] = __hip_atomic_load(&x.data[size_t(sh.pwg) * W + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sh.drow[t] = __hip_atomic_load(&x.diag[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    if (g == 0 && t == 0) {
-        indx[piv_base + j] = piv_base + prow;
-        if (perm && prow != j) {
-            const int a = perm[piv_base + j];
-            perm[piv_base + j] = perm[piv_base + prow];
-            perm[piv_base + prow] = a;
-        }
-    }
+    const int prow = gr;
+    const T* pv = &sh.cd[gw * W];
+    if (g == 0 && t == 0) indx[piv_base + j] = piv_base + prow;
     // (e) exchange rows j and prow
     if (prow != j) {
 #pragma unroll
@@ -244,19 +228,19 @@ __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const 
                 for (int c = 0; c < W; ++c) v[r][c] = sh.drow[c];
             } else if (lr[r] == j) {
 #pragma unroll
-                for (int c = 0; c < W; ++c) v[r][c] = sh.prow[c];
+                for (int c = 0; c < W; ++c) v[r][c] = pv[c];
             }
         }
     }
     // (f) scale by the reciprocal pivot (src/LinAlgLU.pas:264) and update the panel row
-    const T inv = T(1) / sh.prow[j];
+    const T inv = T(1) / pv[j];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         if (lr[r] > j && lr[r] < m) {
             const T l = v[r][j] * inv;
             v[r][j] = l;
 #pragma unroll
-            for (int c = j + 1; c < W; ++c) v[r][c] = fma(-l, sh.prow[c], v[r][c]);
+            for (int c = j + 1; c < W; ++c) v[r][c] = fma(-l, pv[c], v[r][c]);
         }
     }
     return true;
@@ -330,6 +314,29 @@ __global__ void zero_u32_kernel(unsigned* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+// perm from the reference's sequential pivot list: row i of P*A is row perm[i]
+// of A.  One thread replays the n exchanges on a copy held in LDS (dynamic
+// shared memory of n ints) or, for larger n, in global memory.
+__global__ void perm_from_indx_kernel(const int* __restrict__ indx, int* __restrict__ perm, int n, int in_lds) {
+    extern __shared__ int sp[];
+    int* p = in_lds ? sp : perm;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < n; ++i) {
+            const int r = indx[i];
+            if (r != i) {
+                const int a = p[i];
+                p[i] = p[r];
+                p[r] = a;
+            }
+        }
+    }
+    __syncthreads();
+    if (in_lds)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
+}
+
 __global__ void iota_kernel(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = i;
@@ -361,8 +368,11 @@ __global__ void laswp_kernel(T* __restrict__ A, int64_t lda, int k1, int k2, con
 // X = L^-1 X for the R x ncols block X (row stride ldx), L unit lower R x R
 // (row stride ldl); one thread per column, the column in registers.
 template <typename T, int R>
-__global__ void trsm_lunit_block_kernel(const T* __restrict__ L, int64_t ldl, T* __restrict__ X, int64_t ldx,
-                                        int rows, int ncols) {
+__global__ void __launch_bounds__(128)
+trsm_lunit_block_kernel(const T* __restrict__ L, int64_t ldl, T* __restrict__ X, int64_t ldx, int rows, int ncols) {
+    __shared__ T Ls[R][R + 1];
+    for (int e = threadIdx.x; e < rows * rows; e += blockDim.x) Ls[e / rows][e % rows] = L[int64_t(e / rows) * ldl + e % rows];
+    __syncthreads();
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncols) return;
     T x[R];
@@ -373,7 +383,7 @@ __global__ void trsm_lunit_block_kernel(const T* __restrict__ L, int64_t ldl, T*
         if (i < rows) {
             T s = x[i];
 #pragma unroll
-            for (int k = 0; k < i; ++k) s = fma(-L[int64_t(i) * ldl + k], x[k], s);
+            for (int k = 0; k < i; ++k) s = fma(-Ls[i][k], x[k], s);
             x[i] = s;
         }
     }
@@ -385,8 +395,11 @@ __global__ void trsm_lunit_block_kernel(const T* __restrict__ L, int64_t ldl, T*
 // X = U^-1 X for the R x ncols block, U upper with its diagonal (divides, as
 // MatrixLUBackSubst does, src/LinAlgLU.pas:362-384).
 template <typename T, int R>
-__global__ void trsm_upper_block_kernel(const T* __restrict__ U, int64_t ldu, T* __restrict__ X, int64_t ldx,
-                                        int rows, int ncols) {
+__global__ void __launch_bounds__(128)
+trsm_upper_block_kernel(const T* __restrict__ U, int64_t ldu, T* __restrict__ X, int64_t ldx, int rows, int ncols) {
+    __shared__ T Us[R][R + 1];
+    for (int e = threadIdx.x; e < rows * rows; e += blockDim.x) Us[e / rows][e % rows] = U[int64_t(e / rows) * ldu + e % rows];
+    __syncthreads();
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncols) return;
     T x[R];
@@ -398,8 +411,8 @@ __global__ void trsm_upper_block_kernel(const T* __restrict__ U, int64_t ldu, T*
             T s = x[i];
 #pragma unroll
             for (int k = i + 1; k < R; ++k)
-                if (k < rows) s = fma(-U[int64_t(i) * ldu + k], x[k], s);
-            x[i] = s / U[int64_t(i) * ldu + i];
+                if (k < rows) s = fma(-Us[i][k], x[k], s);
+            x[i] = s / Us[i][i];
         }
     }
 #pragma unroll
@@ -444,11 +457,14 @@ namespace mrm {
 template <typename T>
 int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm, LuWork& ws, hipStream_t s) {
     constexpr int W = LU_W;
+    // A grid barrier costs ~2 us with 16 workgroups and ~4.6 us with 64
+    // (measured with the candidate exchange), so rows are packed up to 4 per
+    // thread before more workgroups are added.
     int rpt = 1;
     int G = grid1(m, PANEL_THREADS);
-    if (G > LU_MAX_PANEL_WG) {
-        rpt = 2;
-        G = grid1(m, 2 * PANEL_THREADS);
+    while (G > LU_PANEL_WG_TARGET && rpt < 4) {
+        rpt *= 2;
+        G = grid1(m, rpt * PANEL_THREADS);
     }
     if (G > LU_MAX_PANEL_WG) {
         mrm_set_error(MRM_BAD_DIMS, "lu: matrix too tall for the panel kernel (%d rows)", m);
@@ -459,12 +475,10 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
     uint64_t timeout = 2ull * 100000000ull;  // 2 s of wall clock (100 MHz)
     void* args[] = {&P, &lda, &m, &w, &piv_base, &indx, &perm, &xch, &ws.counter, &ws.status, &timeout};
     hipError_t e;
-    if (rpt == 1)
-        e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 1>), dim3(G),
-                                       dim3(PANEL_THREADS), args, 0, s);
-    else
-        e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 2>), dim3(G),
-                                       dim3(PANEL_THREADS), args, 0, s);
+    const void* fn = rpt == 1   ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 1>)
+                     : rpt == 2 ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 2>)
+                                : reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 4>);
+    e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(PANEL_THREADS), args, 0, s);
     if (e != hipSuccess) {
         mrm_set_error(MRM_HIP_ERROR, "lu panel launch (G=%d): %s", G, hipGetErrorString(e));
         return MRM_HIP_ERROR;
@@ -521,7 +535,6 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
     // indx starts at 0 as in the reference (src/LinAlgLU.pas:289); entries a
     // singular stop leaves unset stay in range for the row interchanges
     zero_i32(indx, n, s);
-    if (perm) hipLaunchKernelGGL(iota_kernel, dim3(grid1(n, 256)), dim3(256), 0, s, perm, n);
     for (int J = 0; J < n; J += LU_NB) {
         const int jb = n - J < LU_NB ? n - J : LU_NB;
         for (int j = J; j < J + jb; j += LU_W) {
@@ -551,6 +564,11 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
                            s);
             if (rc) return rc;
         }
+    }
+    if (perm) {
+        const int in_lds = n <= 32768;
+        hipLaunchKernelGGL(perm_from_indx_kernel, dim3(1), dim3(256), in_lds ? size_t(n) * sizeof(int) : 0, s, indx,
+                           perm, n, in_lds);
     }
     MRM_HIP_TRY(hipGetLastError());
     return MRM_OK;

@@ -106,6 +106,33 @@ MRM_API void mrm_blocked_mult_t2(double* dest, int64_t destLineWidth, const doub
 MRM_API int32_t mrm_dev_dgemm(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                       const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, void* stream);
 
+/* ---- FP64-accurate GEMM from INT8 slices (Ozaki scheme) ------------------ */
+
+/* Device bytes of workspace mrm_dev_dgemm_ozaki needs for these sizes. */
+MRM_API int64_t mrm_ozaki_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_t slices);
+
+/* C(M x N) (op)= opA(A) * opB(B) in FP64, computed on the INT8 matrix cores:
+ * every row of opA(A) and column of opB(B) is scaled by a power of two and cut
+ * into `slices` (1..8) signed 7-bit slices; slice products are summed exactly
+ * in int32 and in FP64 across slices.  Per element the error is at most
+ * 4 (slices+1) K 2^(-7 slices) max|row i of opA(A)| max|column j of opB(B)|
+ * plus FP64 rounding of the slice sums (6-7 slices reach FP64 accuracy).
+ * Inputs must be finite.  `work` is device memory
+ * of at least mrm_ozaki_workspace_bytes(); it must not be shared by calls in
+ * flight at the same time.  Asynchronous on `stream`.  The GEMM the reference
+ * computes with MatrixMult / MatrixMultT1 / MatrixMultT2 (src/MatrixASMStubSwitch.pas
+ * :769-863); the method has no reference counterpart. */
+MRM_API int32_t mrm_dev_dgemm_ozaki(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                                    const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, int32_t slices,
+                                    void* work, int64_t work_bytes, void* stream);
+
+/* Host-pointer form of mrm_dev_dgemm_ozaki with the arguments of
+ * TMatrixBlockedMultfunc (src/MatrixASMStubSwitch.pas:352) plus trans and
+ * slices; the library holds the workspace.  Returns the status. */
+MRM_API int32_t mrm_mult_ozaki(int32_t trans, double* dest, int64_t destLineWidth, const double* mt1,
+                               const double* mt2, int64_t width1, int64_t height1, int64_t width2, int64_t height2,
+                               int64_t LineWidth1, int64_t LineWidth2, int32_t op, int32_t slices);
+
 /* ---- LU factorisation and linear solve ------------------------------------ */
 
 /* MatrixLUDecompInPlace (reference src/LinAlgLU.pas:283-295, recursion :181-281):

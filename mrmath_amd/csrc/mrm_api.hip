@@ -14,6 +14,7 @@
 
 #include "mrm_common.h"
 #include "mrm_host.h"
+#include "mrm_ozaki.h"
 
 namespace {
 
@@ -106,8 +107,9 @@ static int copy_in(void* dst, const void* src, int64_t lw, int64_t w, int64_t h,
 }
 
 // dims: A is (rowsA x colsA) with LW1, B is (rowsB x colsB) with LW2, C is M x N.
+// slices == 0: native FP64 GEMM; 1..8: INT8-slice emulation (workspace in slot 3)
 static int host_gemm(int trans, double* dest, int64_t destLW, const double* mt1, const double* mt2, int64_t w1,
-                     int64_t h1, int64_t w2, int64_t h2, int64_t LW1, int64_t LW2, int op) {
+                     int64_t h1, int64_t w2, int64_t h2, int64_t LW1, int64_t LW2, int op, int slices = 0) {
     clear_error();
     if (w1 == 0 || h1 == 0 || w2 == 0 || h2 == 0) return MRM_OK;  // src/BlockedMult.pas:374-375
     if (w1 < 0 || h1 < 0 || w2 < 0 || h2 < 0 || op < 0 || op > 2) {
@@ -130,6 +132,10 @@ static int host_gemm(int trans, double* dest, int64_t destLW, const double* mt1,
         return MRM_BAD_DIMS;
     }
     if (!dest || !mt1 || !mt2) { mrm_set_error(MRM_BAD_ARG, "null pointer"); return MRM_BAD_ARG; }
+    if (slices < 0 || slices > MRM_OZAKI_MAX_SLICES) {
+        mrm_set_error(MRM_BAD_ARG, "slice count %d outside 1..%d", slices, MRM_OZAKI_MAX_SLICES);
+        return MRM_BAD_ARG;
+    }
     int rc = mrm_ensure_init();
     if (rc) return rc;
 
@@ -142,8 +148,17 @@ static int host_gemm(int trans, double* dest, int64_t destLW, const double* mt1,
     if ((rc = copy_in(dA, mt1, LW1, w1, h1, s))) return rc;
     if ((rc = copy_in(dB, mt2, LW2, w2, h2, s))) return rc;
     if (op != MRM_DO_NONE && (rc = copy_in(dC, dest, destLW, N, M, s))) return rc;
-    rc = mrm_launch_dgemm(trans, M, N, K, static_cast<const double*>(dA), w1 * 8, static_cast<const double*>(dB),
-                          w2 * 8, static_cast<double*>(dC), N * 8, op, s);
+    if (slices == 0) {
+        rc = mrm_launch_dgemm(trans, M, N, K, static_cast<const double*>(dA), w1 * 8, static_cast<const double*>(dB),
+                              w2 * 8, static_cast<double*>(dC), N * 8, op, s);
+    } else {
+        const size_t wb = mrm_ozaki_workspace(M, N, K, slices);
+        void* dW;
+        if ((rc = mrm_staging(3, wb, &dW))) return rc;
+        rc = mrm_launch_dgemm_ozaki(trans, M, N, K, static_cast<const double*>(dA), w1 * 8,
+                                    static_cast<const double*>(dB), w2 * 8, static_cast<double*>(dC), N * 8, op, slices,
+                                    dW, wb, s);
+    }
     if (rc) return rc;
     MRM_HIP_TRY(hipMemcpy2DAsync(dest, size_t(destLW), dC, size_t(N) * 8, size_t(N) * 8, size_t(M),
                                  hipMemcpyDeviceToHost, s));
@@ -216,6 +231,43 @@ int32_t mrm_dev_dgemm(int32_t trans, int64_t M, int64_t N, int64_t K, const doub
         return MRM_BAD_ARG;
     }
     return mrm_launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, mrm_resolve_stream(stream));
+}
+
+int64_t mrm_ozaki_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_t slices) {
+    if (M < 0 || N < 0 || K < 0 || slices < 1 || slices > MRM_OZAKI_MAX_SLICES) return 0;
+    return int64_t(mrm_ozaki_workspace(M, N, K, slices));
+}
+
+int32_t mrm_dev_dgemm_ozaki(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                            const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, int32_t slices,
+                            void* work, int64_t work_bytes, void* stream) {
+    clear_error();
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((M > 0 && N > 0) && (!C || (K > 0 && (!A || !B)))) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    if (work_bytes < 0) work_bytes = 0;
+    return mrm_launch_dgemm_ozaki(trans, M, N, K, A, lda, B, ldb, C, ldc, op, slices, work, size_t(work_bytes),
+                                  mrm_resolve_stream(stream));
+}
+
+int32_t mrm_mult_ozaki(int32_t trans, double* dest, int64_t destLineWidth, const double* mt1, const double* mt2,
+                       int64_t width1, int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1,
+                       int64_t LineWidth2, int32_t op, int32_t slices) {
+    if (trans < 0 || trans > 2) {
+        clear_error();
+        mrm_set_error(MRM_BAD_ARG, "bad trans");
+        return MRM_BAD_ARG;
+    }
+    if (slices < 1) {
+        clear_error();
+        mrm_set_error(MRM_BAD_ARG, "slice count %d outside 1..%d", slices, MRM_OZAKI_MAX_SLICES);
+        return MRM_BAD_ARG;
+    }
+    return host_gemm(trans, dest, destLineWidth, mt1, mt2, width1, height1, width2, height2, LineWidth1, LineWidth2,
+                     op, slices);
 }
 
 }  // extern "C"

@@ -68,6 +68,20 @@ def MatrixMultT1Ex(dest, mt1, mt2, blockSize: int = BlockMatrixCacheSize, op: in
     return _call_mult(_lib.load().mrm_blocked_mult_t1, dest, mt1, mt2, blockSize, op, None)
 
 
+def MatrixMultOzaki(dest, mt1, mt2, slices: int = 7, trans: int = _lib.NN, op: int = doNone):
+    """dest (op)= op(mt1) * op(mt2) in FP64, computed from `slices` 7-bit INT8
+    slices per operand on the INT8 matrix cores (trans: NN = MatrixMult,
+    TN = MatrixMultT1, NT = MatrixMultT2 dimension rules).  Error per element
+    <= 4 (slices+1) K 2^(-7 slices) max|row of op(mt1)| max|column of op(mt2)|
+    plus FP64 rounding.  No reference counterpart (the reference multiplies in
+    FP64 only)."""
+    d, _, _, dlw = _mat(dest, "dest")
+    p1, w1, h1, lw1 = _mat(mt1, "mt1")
+    p2, w2, h2, lw2 = _mat(mt2, "mt2")
+    _lib.check(_lib.load().mrm_mult_ozaki(trans, d, dlw, p1, p2, w1, h1, w2, h2, lw1, lw2, op, slices))
+    return dest
+
+
 def MatrixMultT2(dest, mt1, mt2):
     """dest = mt1 * mt2^T via multT2Func (src/MatrixASMStubSwitch.pas:853-858)."""
     return _call_mult(_lib.load().mrm_mult_t2, dest, mt1, mt2)

@@ -37,6 +37,7 @@ _SIG = {
     "orc_generic_mtx_mult_transp": (None, [_p, _i64, _p, _p, _i64, _i64, _i64, _i64, _i64, _i64]),
     "orc_generic_transp_mtx_mult": (None, [_p, _i64, _p, _p, _i64, _i64, _i64, _i64, _i64, _i64]),
     "orc_blocked_mult": (None, [_i32, _p, _i64, _p, _p, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32]),
+    "orc_ozaki_gemm": (_i32, [_p, _i64, _p, _i64, _p, _i64, _i64, _i64, _i64, _i32, _i32, _i64]),
     "orc_lu_decomp_inplace": (_i32, [_p, _i64, _i64, _p]),
     "orc_lu_backsubst": (None, [_p, _i64, _i64, _p, _p, _i64]),
     "orc_lineq_solve": (_i32, [_p, _i64, _i64, _p, _i64, _p, _i64, _i64, _i64]),
@@ -179,6 +180,27 @@ def gemm_err(Cm: np.ndarray, Cref: np.ndarray, A: np.ndarray, B: np.ndarray) -> 
     pa, wa, ha, lwa = _m(A)
     pb, wb, hb, lwb = _m(B)
     return lib().orc_gemm_normwise_err(pc, lwc, pr, lwr, wc, hc, pa, lwa, wa, ha, pb, lwb, wb, hb)
+
+
+OZAKI_KCHUNK = 16384
+
+
+def ozaki_gemm(A: np.ndarray, B: np.ndarray, slices: int, C: np.ndarray | None = None, op: int = 0,
+               kchunk: int = OZAKI_KCHUNK) -> np.ndarray:
+    """C (op)= A @ B through `slices` 7-bit integer slices (the GPU's method,
+    bit for bit).  Raises if an int32 chunk sum would overflow."""
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    if C is None:
+        C = np.zeros((M, N))
+    pc, _, _, lwc = _m(C)
+    pa, _, _, lwa = _m(A)
+    pb, _, _, lwb = _m(B)
+    rc = lib().orc_ozaki_gemm(pc, lwc, pa, lwa, pb, lwb, M, N, K, op, slices, kchunk)
+    if rc:
+        raise ValueError(f"orc_ozaki_gemm failed ({rc})")
+    return C
 
 
 # ---- LU / solve ---------------------------------------------------------------

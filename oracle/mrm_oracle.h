@@ -23,6 +23,9 @@
  *                            det / inv on random inputs, singular detection.
  *   orc_randgauss            algorithm only (Marsaglia polar); pinned through
  *                            the MT stream plus the closed-form transform.
+ *   orc_ozaki_gemm           exact products of integer inputs the slices hold
+ *                            completely, the a-priori error bound and its
+ *                            2^-7 decay per slice, a numpy digit split.
  *   orc_lineq_solve with NumRefinments > 0: "parity unpinned" -- the reference
  *                            has no test that runs refinement.
  */
@@ -87,6 +90,11 @@ double  orc_determinant(const double* A, int64_t LWA, int64_t width);
 double orc_gemm_normwise_err(const double* C, int64_t LWC, const double* Cref, int64_t LWR,
                              int64_t wC, int64_t hC, const double* A, int64_t LWA, int64_t wA, int64_t hA,
                              const double* B, int64_t LWB, int64_t wB, int64_t hB);
+/* FP64 GEMM C (M x N) (op)= A (M x K) * B (K x N) through `slices` 7-bit
+ * integer slices per operand, accumulated exactly per diagonal and K chunk
+ * and in FP64 across them (mrm_ozaki_oracle.c; no reference counterpart). */
+int32_t orc_ozaki_gemm(double* C, int64_t LWC, const double* A, int64_t LWA, const double* B, int64_t LWB,
+                       int64_t M, int64_t N, int64_t K, int32_t op, int32_t slices, int64_t kchunk);
 /* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf), residual in long double. */
 double orc_backward_error(const double* A, int64_t LWA, int64_t n, const double* X, int64_t LWX,
                           const double* B, int64_t LWB, int64_t nrhs);

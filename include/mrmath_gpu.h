@@ -54,6 +54,12 @@ extern "C" {
 #define MRM_TN 1 /* C = A^T * B (MultT1) */
 #define MRM_NT 2 /* C = A * B^T (MultT2) */
 
+/* Factorisation precisions of the linear solvers: plain FP64, or the trailing
+ * updates on the FP32 / BF16 matrix cores with FP64 iterative refinement. */
+#define MRM_SOLVE_FP64 0
+#define MRM_SOLVE_FP32_IR 1
+#define MRM_SOLVE_BF16_IR 2
+
 /* ---- library state ------------------------------------------------------ */
 
 /* Selects the current HIP device, creates the library stream.  Called
@@ -162,6 +168,43 @@ MRM_API int32_t mrm_dev_lu_factor(double* A, int64_t lda, int64_t n, int32_t* in
  * mrm_dev_lu_factor.  B and X must not overlap.  Asynchronous. */
 MRM_API int32_t mrm_dev_lu_solve(const double* LU, int64_t lda, int64_t n, const int32_t* perm, const double* B,
                                  int64_t ldb, double* X, int64_t ldx, int64_t nrhs, void* stream);
+
+/* ---- mixed precision: low-precision update and LU-IR solve ---------------- */
+
+/* Device bytes of packed operands mrm_dev_gemm_lowp needs. */
+MRM_API int64_t mrm_lowp_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_t mode);
+
+/* C (M x N, FP64) -= A (M x K) * B (K x N), FP64 operands in device memory
+ * (byte strides), the product rounded to FP32 (mode MRM_SOLVE_FP32_IR) or BF16
+ * (MRM_SOLVE_BF16_IR) and accumulated in FP32 on the matrix cores.  The
+ * Schur-complement update of the mixed-precision LU (the reference computes it
+ * in FP64 through the blocked multiply, src/LinAlgLU.pas:211-227).
+ * Asynchronous on `stream`; `work` must not be shared by calls in flight. */
+MRM_API int32_t mrm_dev_gemm_lowp(int32_t mode, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                                  const double* B, int64_t ldb, double* C, int64_t ldc, void* work,
+                                  int64_t work_bytes, void* stream);
+
+/* MatrixLinEQSolve on device memory (reference src/LinAlgLU.pas:481-592) with
+ * a choice of factorisation precision: X (n x nrhs) = A^-1 B.  The LU of a copy
+ * of A runs in `mode` (MRM_SOLVE_FP64 / _FP32_IR / _BF16_IR: trailing updates
+ * on the FP32 / BF16 matrix cores, FP64 panels and pivoting), then FP64
+ * refinement x += (LU)^-1 (b - A x) runs until the normwise backward error
+ * max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= tol or
+ * max_iter steps.  A low-precision run whose refinement stalls is redone in
+ * FP64 (*fell_back = 1).  Returns MRM_OK, MRM_SINGULAR or MRM_NOT_CONVERGED
+ * (X then holds the last iterate).  iterations / backward_error / fell_back
+ * may be NULL.  Synchronises `stream`. */
+MRM_API int32_t mrm_dev_lineq_solve_ir(int32_t mode, const double* A, int64_t lda, int64_t n, const double* B,
+                                       int64_t ldb, double* X, int64_t ldx, int64_t nrhs, int32_t max_iter,
+                                       double tol, int32_t* iterations, double* backward_error, int32_t* fell_back,
+                                       void* stream);
+
+/* MatrixLinEQSolve (src/LinAlgLU.pas:481-592) with a factorisation precision.
+ * mode MRM_SOLVE_FP64 is mrm_lineq_solve; the IR modes refine to a backward
+ * error of 1e-14 in at most NumRefinments steps (50 when 0). */
+MRM_API int32_t mrm_lineq_solve_ex(const double* A, int64_t LineWidthA, int64_t width, const double* B,
+                                   int64_t LineWidthB, double* X, int64_t LineWidthX, int64_t width2,
+                                   int64_t NumRefinments, int32_t mode);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libmrmath_gpu.so"
 MRM_OK, MRM_SINGULAR, MRM_BAD_DIMS, MRM_BAD_ARG, MRM_HIP_ERROR, MRM_NO_MEMORY, MRM_NOT_INITIALISED, MRM_NOT_CONVERGED = range(8)
 DO_NONE, DO_ADD, DO_SUB = 0, 1, 2
 NN, TN, NT = 0, 1, 2
+SOLVE_FP64, SOLVE_FP32_IR, SOLVE_BF16_IR = 0, 1, 2
 
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -36,6 +37,11 @@ SIGNATURES = {
     "mrm_dev_dgemm_ozaki": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _i32, _pv, _i64,
                                    _pv]),
     "mrm_mult_ozaki": (_i32, [_i32, _pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _i32]),
+    "mrm_lowp_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32]),
+    "mrm_dev_gemm_lowp": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _pv, _i64, _pv]),
+    "mrm_dev_lineq_solve_ir": (_i32, [_i32, _pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i32, C.c_double, _pv, _pv,
+                                      _pv, _pv]),
+    "mrm_lineq_solve_ex": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64, _i32]),
     "mrm_lu_decomp_inplace": (_i32, [_pd, _i64, _i64, _pv]),
     "mrm_lu_backsubst": (_i32, [_pd, _i64, _i64, _pv, _pd, _i64]),
     "mrm_lineq_solve": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64]),

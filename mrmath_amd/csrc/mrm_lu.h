@@ -20,6 +20,8 @@ struct LuWork {
     unsigned* counter = nullptr;  // grid-barrier arrivals
     int* status = nullptr;        // MRM_OK / MRM_SINGULAR / MRM_LU_TIMEOUT
     void* xch = nullptr;          // pivot exchange area
+    void* lowp = nullptr;         // packed operands of the low-precision trailing update
+    size_t lowp_bytes = 0;
 };
 
 size_t lu_xch_bytes();
@@ -37,8 +39,15 @@ inline int gemm_t<double>(int trans, int64_t M, int64_t N, int64_t K, const doub
 
 // In-place LU with partial pivoting of the n x n matrix A (element stride lda).
 // indx: reference pivot format; perm (optional): row i of P*A is row perm[i] of A.
+// mode MRM_SOLVE_FP32_IR / MRM_SOLVE_BF16_IR computes the outer trailing
+// updates in that precision (ws.lowp must hold lu_lowp_bytes(n, mode)).
 template <typename T>
-int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s);
+int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode = MRM_SOLVE_FP64);
+
+// Packed-operand bytes of the low-precision trailing updates of an n x n LU.
+size_t lu_lowp_bytes(int n, int mode);
+// Grows ws.lowp to lu_lowp_bytes(n, mode); caller holds mrm_host_mutex().
+int lu_lowp_reserve(LuWork& ws, int n, int mode);
 
 // X = (LU)^-1 P B for nrhs columns.
 template <typename T>

@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "mrm_common.h"
+#include "mrm_lowp.h"
 #include "mrm_lu.h"
 
 namespace {
@@ -531,7 +532,7 @@ int trsm_upper(const T* U, int64_t ldu, T* X, int64_t ldx, int rows, int ncols, 
 }
 
 template <typename T>
-int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s) {
+int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode) {
     // indx starts at 0 as in the reference (src/LinAlgLU.pas:289); entries a
     // singular stop leaves unset stay in range for the row interchanges
     zero_i32(indx, n, s);
@@ -559,9 +560,14 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
             int rc = trsm_lower_unit<T>(A + int64_t(J) * lda + J, lda, A + int64_t(J) * lda + J + jb, lda, jb,
                                         n - J - jb, s);
             if (rc) return rc;
-            rc = gemm_t<T>(MRM_NN, n - J - jb, n - J - jb, jb, A + int64_t(J + jb) * lda + J, lda,
-                           A + int64_t(J) * lda + J + jb, lda, A + int64_t(J + jb) * lda + J + jb, lda, MRM_DO_SUB,
-                           s);
+            if (mode == MRM_SOLVE_FP64)
+                rc = gemm_t<T>(MRM_NN, n - J - jb, n - J - jb, jb, A + int64_t(J + jb) * lda + J, lda,
+                               A + int64_t(J) * lda + J + jb, lda, A + int64_t(J + jb) * lda + J + jb, lda,
+                               MRM_DO_SUB, s);
+            else
+                rc = mrm_launch_lowp_update(mode, n - J - jb, n - J - jb, jb, A + int64_t(J + jb) * lda + J, lda,
+                                            A + int64_t(J) * lda + J + jb, lda, A + int64_t(J + jb) * lda + J + jb,
+                                            lda, ws.lowp, ws.lowp_bytes, s);
             if (rc) return rc;
         }
     }
@@ -589,7 +595,26 @@ int lu_solve(const T* LU, int64_t lda, int n, const int* perm, const T* B, int64
     return MRM_OK;
 }
 
-template int lu_factor<double>(double*, int64_t, int, int*, int*, LuWork&, hipStream_t);
+template int lu_factor<double>(double*, int64_t, int, int*, int*, LuWork&, hipStream_t, int);
+
+size_t lu_lowp_bytes(int n, int mode) {
+    if (mode == MRM_SOLVE_FP64 || n <= LU_NB) return 0;
+    return mrm_lowp_workspace(n - LU_NB, n - LU_NB, LU_NB, mode);
+}
+
+int lu_lowp_reserve(LuWork& ws, int n, int mode) {
+    const size_t need = lu_lowp_bytes(n, mode);
+    if (need <= ws.lowp_bytes) return MRM_OK;
+    if (ws.lowp) MRM_HIP_TRY(hipFree(ws.lowp));
+    ws.lowp = nullptr;
+    ws.lowp_bytes = 0;
+    if (hipMalloc(&ws.lowp, need) != hipSuccess) {
+        mrm_set_error(MRM_NO_MEMORY, "lu: cannot allocate %zu bytes of packed-operand workspace", need);
+        return MRM_NO_MEMORY;
+    }
+    ws.lowp_bytes = need;
+    return MRM_OK;
+}
 template int lu_solve<double>(const double*, int64_t, int, const int*, const double*, int64_t, double*, int64_t, int,
                               hipStream_t);
 

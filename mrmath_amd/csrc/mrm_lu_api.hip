@@ -9,6 +9,8 @@
 
 #include "mrm_common.h"
 #include "mrm_host.h"
+#include "mrm_ir.h"
+#include "mrm_lowp.h"
 #include "mrm_lu.h"
 
 namespace mrm {
@@ -220,6 +222,97 @@ int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, cons
                                  hipMemcpyDeviceToHost, s));
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return MRM_OK;
+}
+
+int64_t mrm_lowp_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_t mode) {
+    if (M < 0 || N < 0 || K < 0 || (mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR)) return 0;
+    return int64_t(mrm_lowp_workspace(M, N, K, mode));
+}
+
+int32_t mrm_dev_gemm_lowp(int32_t mode, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                          const double* B, int64_t ldb, double* C, int64_t ldc, void* work, int64_t work_bytes,
+                          void* stream) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if (M < 0 || N < 0 || K < 0 || ((lda | ldb | ldc) & 7) || (M > 0 && K > 0 && lda < K * 8) ||
+        (K > 0 && N > 0 && ldb < N * 8) || (M > 0 && N > 0 && ldc < N * 8)) {
+        mrm_set_error(MRM_BAD_DIMS, "lowp update: bad dimensions or strides");
+        return MRM_BAD_DIMS;
+    }
+    if (M > 0 && N > 0 && K > 0 && (!A || !B || !C)) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    return mrm_launch_lowp_update(mode, M, N, K, A, lda / 8, B, ldb / 8, C, ldc / 8, work,
+                                  work_bytes < 0 ? 0 : size_t(work_bytes), mrm_resolve_stream(stream));
+}
+
+int32_t mrm_dev_lineq_solve_ir(int32_t mode, const double* A, int64_t lda, int64_t n, const double* B, int64_t ldb,
+                               double* X, int64_t ldx, int64_t nrhs, int32_t max_iter, double tol,
+                               int32_t* iterations, double* backward_error, int32_t* fell_back, void* stream) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(n, lda))) return rc;
+    if (nrhs < 0 || nrhs > 0x7fffffffLL || ldb < nrhs * 8 || ldx < nrhs * 8 || ((ldb | ldx) & 7) || max_iter < 0 ||
+        !(tol >= 0.0)) {
+        mrm_set_error(MRM_BAD_DIMS, "solve: bad right-hand-side layout, max_iter or tol");
+        return MRM_BAD_DIMS;
+    }
+    if (n > 0 && nrhs > 0 && (!A || !B || !X)) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    mrm::IrStats st;
+    rc = mrm::lineq_solve_ir(mode, A, lda / 8, int(n), B, ldb / 8, X, ldx / 8, int(nrhs), max_iter, tol, &st,
+                             mrm_resolve_stream(stream));
+    if (iterations) *iterations = st.iterations;
+    if (backward_error) *backward_error = st.backward_error;
+    if (fell_back) *fell_back = st.fell_back;
+    return rc;
+}
+
+int32_t mrm_lineq_solve_ex(const double* A, int64_t LineWidthA, int64_t width, const double* B, int64_t LineWidthB,
+                           double* X, int64_t LineWidthX, int64_t width2, int64_t NumRefinments, int32_t mode) {
+    if (mode == MRM_SOLVE_FP64)
+        return mrm_lineq_solve(A, LineWidthA, width, B, LineWidthB, X, LineWidthX, width2, NumRefinments);
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(width, LineWidthA))) return rc;
+    if (width2 < 0 || LineWidthB < width2 * 8 || LineWidthX < width2 * 8 || ((LineWidthB | LineWidthX) & 7) ||
+        NumRefinments < 0) {
+        mrm_set_error(MRM_BAD_DIMS, "Dimension error");
+        return MRM_BAD_DIMS;
+    }
+    if (mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR) {
+        mrm_set_error(MRM_BAD_ARG, "solve: unknown mode %d", mode);
+        return MRM_BAD_ARG;
+    }
+    if (width == 0 || width2 == 0) return MRM_OK;
+    if (!A || !B || !X) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    const int64_t n = width, m = width2;
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    void *dA, *dB, *dX;
+    if ((rc = mrm_staging(0, size_t(n * n) * 8, &dA))) return rc;
+    if ((rc = mrm_staging(2, size_t(n * m) * 8, &dB))) return rc;
+    if ((rc = mrm_staging(3, size_t(n * m) * 8, &dX))) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    MRM_HIP_TRY(hipMemcpy2DAsync(dB, size_t(m) * 8, B, size_t(LineWidthB), size_t(m) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    mrm::IrStats st;
+    rc = mrm::lineq_solve_ir(mode, static_cast<const double*>(dA), n, int(n), static_cast<const double*>(dB), m,
+                             static_cast<double*>(dX), m, int(m), NumRefinments ? int(NumRefinments) : 50, 1e-14,
+                             &st, s);
+    if (rc != MRM_OK && rc != MRM_NOT_CONVERGED) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(X, size_t(LineWidthX), dX, size_t(m) * 8, size_t(m) * 8, size_t(n),
+                                 hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    return rc;
 }
 
 }  // extern "C"

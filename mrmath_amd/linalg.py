@@ -13,6 +13,7 @@ from . import _lib
 from .stubswitch import _mat
 
 leOk, leSingular = _lib.MRM_OK, _lib.MRM_SINGULAR
+smFP64, smFP32IR, smBF16IR = _lib.SOLVE_FP64, _lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR
 
 
 class ELinEQSingularException(Exception):
@@ -54,6 +55,21 @@ def MatrixLinEQSolve(A: np.ndarray, B: np.ndarray, X: np.ndarray, NumRefinments:
     if w != h or hb != w or hx != w or wx != w2:
         raise ValueError("Dimension error")
     return _status(_lib.load().mrm_lineq_solve(pa, lwa, w, pb, lwb, px, lwx, w2, NumRefinments))
+
+
+def MatrixLinEQSolveEx(A: np.ndarray, B: np.ndarray, X: np.ndarray, NumRefinments: int = 0,
+                       mode: int = smFP64) -> int:
+    """MatrixLinEQSolve with a factorisation precision: smFP64 is
+    MatrixLinEQSolve; smFP32IR / smBF16IR run the LU's trailing updates on the
+    FP32 / BF16 matrix cores and refine in FP64 to a normwise backward error of
+    1e-14 (at most NumRefinments steps, 50 when 0).  Returns leOk or
+    leSingular; raises MrmError when refinement does not converge."""
+    pa, w, h, lwa = _mat(A, "A")
+    pb, w2, hb, lwb = _mat(B, "B")
+    px, wx, hx, lwx = _mat(X, "X")
+    if w != h or hb != w or hx != w or wx != w2:
+        raise ValueError("Dimension error")
+    return _status(_lib.load().mrm_lineq_solve_ex(pa, lwa, w, pb, lwb, px, lwx, w2, NumRefinments, mode))
 
 
 def SolveLinEQ(A: np.ndarray, B: np.ndarray, NumRefinments: int = 0) -> np.ndarray:

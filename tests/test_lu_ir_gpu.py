@@ -1,0 +1,141 @@
+"""GPU tests of the mixed-precision path: the low-precision trailing update
+(mrm_dev_gemm_lowp) and the LU with FP64 iterative refinement
+(mrm_dev_lineq_solve_ir, mrm_lineq_solve_ex).
+
+Bars: the update is exact on integer inputs the low precision represents and
+within its rounding bound otherwise; every solve reaches the normwise backward
+error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= 1e-14
+(recomputed here in long double by the oracle), and its solution agrees with
+the oracle's FP64 solve to the conditioning: ||x - x_oracle|| / ||x|| <=
+kappa * 1e-13.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from mrmath_amd import _lib
+from mrmath_amd import linalg as la
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _lowp_update(mode, A, B, C0):
+    import torch
+    lib = _lib.load()
+    M, K = A.shape
+    N = B.shape[1]
+    dA, dB, dC = _dev(A), _dev(B), _dev(C0)
+    wb = lib.mrm_lowp_workspace_bytes(M, N, K, mode)
+    W = torch.empty(max(wb, 1), dtype=torch.uint8, device="cuda")
+    rc = lib.mrm_dev_gemm_lowp(mode, M, N, K, dA.data_ptr(), K * 8, dB.data_ptr(), N * 8, dC.data_ptr(), N * 8,
+                               W.data_ptr(), wb, torch.cuda.current_stream().cuda_stream)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return dC.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (300, 257, 256), (513, 260, 100)])
+def test_lowp_update_exact_on_small_integers(mode, M, N, K):
+    """Integers in [-8, 8] are exact in BF16 and FP32 and their sums of products
+    stay below 2^24, so C - A B must come out exactly."""
+    rng = np.random.default_rng(M * N + K)
+    A = rng.integers(-8, 9, (M, K)).astype(np.float64)
+    B = rng.integers(-8, 9, (K, N)).astype(np.float64)
+    C0 = rng.integers(-1000, 1000, (M, N)).astype(np.float64)
+    assert np.array_equal(_lowp_update(mode, A, B, C0), C0 - orc.generic_mult(A, B))
+
+
+@pytest.mark.parametrize("mode,u", [(_lib.SOLVE_FP32_IR, 2.0 ** -24), (_lib.SOLVE_BF16_IR, 2.0 ** -8)])
+def test_lowp_update_rounding_bound(mode, u):
+    M, N, K = 700, 600, 256
+    A = orc.pm1_matrix(K, M, 11)
+    B = orc.pm1_matrix(N, K, 12)
+    C0 = orc.pm1_matrix(N, M, 13)
+    got = _lowp_update(mode, A, B, C0)
+    ref = C0 - orc.generic_mult(A, B)
+    bound = (2 * u + K * 2.0 ** -24) * (np.abs(A) @ np.abs(B)) + 4 * 2.0 ** -53 * np.abs(C0)
+    err = np.abs(got - ref)
+    assert np.all(err <= bound), float(np.max(err / bound))
+    # and it is a low-precision product, not an FP64 one
+    assert np.max(err) > 1e-12
+
+
+def _system(n, kappa, nrhs, seed):
+    rng = np.random.default_rng(seed)
+    Uq, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Vq, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.logspace(0, -np.log10(kappa), n)
+    A = (Uq * s) @ Vq.T
+    B = rng.standard_normal((n, nrhs))
+    return np.ascontiguousarray(A), B
+
+
+def _solve_ir(mode, A, B, max_iter=50, tol=1e-14):
+    import torch
+    lib = _lib.load()
+    n, nrhs = B.shape
+    dA, dB = _dev(A), _dev(B)
+    dX = torch.zeros(n, nrhs, dtype=torch.float64, device="cuda")
+    it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
+    rc = lib.mrm_dev_lineq_solve_ir(mode, dA.data_ptr(), n * 8, n, dB.data_ptr(), nrhs * 8, dX.data_ptr(), nrhs * 8,
+                                    nrhs, max_iter, tol, C.byref(it), C.byref(be), C.byref(fb),
+                                    torch.cuda.current_stream().cuda_stream)
+    return rc, dX.cpu().numpy(), it.value, be.value, fb.value
+
+
+@pytest.mark.parametrize("n,kappa,nrhs", [(257, 1e2, 3), (1000, 1e6, 8), (2500, 1e2, 64), (2500, 1e6, 64)])
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR])
+def test_lu_ir_reaches_fp64_backward_error(n, kappa, nrhs, mode):
+    A, B = _system(n, kappa, nrhs, int(n + np.log10(kappa)))
+    rc, X, it, be, fb = _solve_ir(mode, A, B)
+    print(f"mode={mode} n={n} kappa={kappa:g}: rc={rc} steps={it} berr={be:.3e} fell_back={fb}")
+    assert rc == 0
+    be_oracle = orc.backward_error(A, X, B)
+    assert be <= 1e-14 and be_oracle <= 1e-14, (be, be_oracle)
+    if mode == _lib.SOLVE_FP32_IR:
+        assert fb == 0  # FP32 updates suffice up to kappa 1e6
+    rc, Xo = orc.lineq_solve(A, B, 0)
+    rel = np.linalg.norm(X - Xo) / np.linalg.norm(Xo)
+    assert rel <= kappa * 1e-13, rel
+
+
+def test_lu_ir_ill_conditioned_falls_back_or_converges():
+    """kappa = 1e10 is beyond what FP32 factors can refine; the solve must still
+    return a 1e-14 backward-error solution, through the FP64 fallback if needed."""
+    A, B = _system(800, 1e10, 4, 99)
+    for mode in (_lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR):
+        rc, X, it, be, fb = _solve_ir(mode, A, B)
+        print(f"mode={mode} kappa=1e10: rc={rc} steps={it} berr={be:.3e} fell_back={fb}")
+        assert rc == 0
+        assert orc.backward_error(A, X, B) <= 1e-14
+
+
+def test_lu_ir_singular_and_errors():
+    lib = _lib.load()
+    A = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [1.0, 1.0, 1.0]])
+    rc, X, it, be, fb = _solve_ir(_lib.SOLVE_FP32_IR, A, np.ones((3, 1)))
+    assert rc == _lib.MRM_SINGULAR
+    assert lib.mrm_dev_lineq_solve_ir(7, None, 8, 1, None, 8, None, 8, 1, 5, 1e-14, None, None, None, None) != 0
+    # max_iter = 0 on an ill-conditioned system with BF16 factors cannot converge in FP64 either
+    Ai, Bi = _system(300, 1e10, 2, 5)
+    rc, X, it, be, fb = _solve_ir(_lib.SOLVE_BF16_IR, Ai, Bi, max_iter=0, tol=1e-30)
+    assert rc == _lib.MRM_NOT_CONVERGED
+
+
+@pytest.mark.parametrize("mode", [la.smFP32IR, la.smBF16IR])
+def test_host_entry_with_mode(mode):
+    A, B = _system(600, 1e3, 5, 17)
+    X = np.zeros_like(B)
+    assert la.MatrixLinEQSolveEx(A, B, X, 0, mode) == la.leOk
+    assert orc.backward_error(A, X, B) <= 1e-14
+    X2 = np.zeros_like(B)
+    assert la.MatrixLinEQSolveEx(A, B, X2, 1, la.smFP64) == la.leOk
+    assert np.allclose(X, X2, rtol=1e-9, atol=1e-9)

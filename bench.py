@@ -1,16 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark of the FP64 GEMM hot path (mrMath's TDoubleMatrix.Mult / MatrixMult)
-on one MI355X.
+"""Benchmark of mrMath's FP64 multiply / LU-solve hot path on one MI355X.
 
 One step = one C = A*B on device-resident 8192 x 8192 float64 operands
-(entries uniform [-1, 1)) through the library's device entry, i.e. the
-square case of the GPU GEMM family.  Prints one JSON line.
+(entries uniform [-1, 1)) through the library's device entry (the square case
+of the GPU GEMM family).  Prints one JSON line; besides the headline GEMM it
+carries the other measured rows of the path under "rows":
 
-  python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+  dgemm_tn_4096 / dgemm_nt_4096   MultT1 / MultT2 at 4096^3
+  syrk_200000x512                 X^T X with X 200000 x 512 (split-K GEMM)
+  ozaki_8192                      FP64 GEMM emulated with INT8 slices, s = 3..7,
+                                  wide-range inputs (2 res53 - 1) 2^e, e in [-20, 20]
+  lu_ir_16384                     A X = B, n = 16384, 64 right-hand sides,
+                                  A = U diag(s) V^T with kappa 1e2 / 1e6 / 1e10:
+                                  plain FP64 LU vs FP32- and BF16-update LU with
+                                  FP64 refinement to backward error 1e-14
 
-Workloads: dgemm_nn_8192 (default), dgemm_tn_4096, dgemm_nt_4096, and
-copy_1gib (a plain 1 GiB device copy: the calibration kernel for the HBM byte
-counters FETCH_SIZE / WRITE_SIZE).
+  python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-rows]
+
+--workload runs one workload alone (dgemm_nn_8192 default, dgemm_tn_4096,
+dgemm_nt_4096, syrk_200000x512, ozaki_8192, lu_ir_16384, and copy_1gib: a
+plain 1 GiB device copy, the calibration kernel for the HBM byte counters).
 """
 from __future__ import annotations
 
@@ -23,6 +32,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -33,7 +44,9 @@ WORKLOADS = {
     "dgemm_nn_8192": (0, 8192, 8192, 8192),
     "dgemm_tn_4096": (1, 4096, 4096, 4096),
     "dgemm_nt_4096": (2, 4096, 4096, 4096),
+    "syrk_200000x512": (1, 512, 512, 200000),
 }
+INT8_PEAK_TOPS = 5000.0
 
 
 def _traffic_from_profile(workload: str):
@@ -93,13 +106,174 @@ def copy_calibration(args):
                       "ms_per_copy": round(ms, 4), "GBps": round(2 * (1 << 30) / ms / 1e6, 1)}), flush=True)
 
 
+def gemm_run(lib, workload: str, steps: int, warmup: int):
+    """Times the device GEMM of `workload` with HIP events around every launch
+    on the stream it runs on; checks C x == A (B x) outside the timed region."""
+    import torch
+    from mrmath_amd import _lib
+
+    trans, M, N, K = WORKLOADS[workload]
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(8101)
+    rowsA, colsA = (K, M) if trans == 1 else (M, K)
+    rowsB, colsB = (N, K) if trans == 2 else (K, N)
+    A = torch.rand(rowsA, colsA, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    if workload.startswith("syrk"):
+        B, colsB = A, colsA  # X^T X
+    else:
+        B = torch.rand(rowsB, colsB, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    C = torch.empty(M, N, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def step():
+        rc = lib.mrm_dev_dgemm(trans, M, N, K, A.data_ptr(), colsA * 8, B.data_ptr(), colsB * 8,
+                               C.data_ptr(), N * 8, 0, sp)
+        if rc:
+            _lib.check(rc)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s_, e_ in ev:
+        s_.record(stream)
+        step()
+        e_.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [s_.elapsed_time(e_) for s_, e_ in ev]
+
+    x = torch.randn(N, dtype=torch.float64, device=dev, generator=g)
+    Am = A.t() if trans == 1 else A
+    Bm = B.t() if trans == 2 else B
+    rel = (torch.linalg.vector_norm(C @ x - Am @ (Bm @ x)) /
+           (torch.linalg.vector_norm(A) * torch.linalg.vector_norm(B) * torch.linalg.vector_norm(x))).item()
+    if not rel <= 4 * K * 2.0 ** -53:
+        print(json.dumps({"error": f"{workload}: GEMM result failed the linearity check", "rel": rel}),
+              file=sys.stderr)
+        sys.exit(1)
+    flops = 2.0 * M * N * K
+    avg_ms = statistics.mean(kernel_ms)
+    return {
+        "M": M, "N": N, "K": K, "op": ["NN", "TN (MultT1)", "NT (MultT2)"][trans],
+        "ms_per_step": wall * 1e3 / steps,
+        "tflops": flops / (wall / steps) / 1e12,
+        "avg_launch_ms": avg_ms,
+        "achieved_tflops": flops / (avg_ms * 1e-3) / 1e12,
+        "linearity_rel_err": rel,
+    }
+
+
+def ozaki_row(lib, reps: int = 3):
+    """FP64 GEMM emulated with s = 3..7 INT8 slices at 8192^3 on wide-range
+    inputs, against the native FP64 MFMA GEMM of the same operands (itself
+    checked against the oracle in the tests)."""
+    import torch
+
+    n = 8192
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(3101)
+
+    def wide():
+        u = torch.rand(n, n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+        return u * torch.exp2(torch.randint(-20, 21, (n, n), device=dev, generator=g).double())
+
+    A, B = wide(), wide()
+    sp = torch.cuda.current_stream().cuda_stream
+    Cd = torch.empty(n, n, dtype=torch.float64, device=dev)
+    lib.mrm_dev_dgemm(0, n, n, n, A.data_ptr(), n * 8, B.data_ptr(), n * 8, Cd.data_ptr(), n * 8, 0, sp)
+    na = float(torch.linalg.norm(A)) * float(torch.linalg.norm(B))
+    wb = lib.mrm_ozaki_workspace_bytes(n, n, n, 7)
+    W = torch.empty(wb, dtype=torch.uint8, device=dev)
+    C = torch.empty_like(Cd)
+    bar = 4 * n * 2.0 ** -53
+    per_s = {}
+    for s in range(3, 8):
+        def call():
+            rc = lib.mrm_dev_dgemm_ozaki(0, n, n, n, A.data_ptr(), n * 8, B.data_ptr(), n * 8, C.data_ptr(), n * 8,
+                                         0, s, W.data_ptr(), wb, sp)
+            assert rc == 0, rc
+        call()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = statistics.median(ts)
+        err = float(torch.linalg.norm(C - Cd)) / na
+        per_s[str(s)] = {"ms": round(ms, 3), "fp64_equiv_tflops": round(2.0 * n ** 3 / ms / 1e9, 2),
+                         "int8_tops": round(s * (s + 1) // 2 * 2.0 * n ** 3 / ms / 1e9, 1),
+                         "normwise_err_vs_fp64_gemm": err, "meets_fp64_bar": err <= bar}
+    ok = [s for s in per_s if per_s[s]["meets_fp64_bar"]]
+    best = max(ok, key=lambda s: per_s[s]["fp64_equiv_tflops"]) if ok else None
+    return {
+        "config": "8192^3, entries (2 res53 - 1) 2^e, e uniform in [-20, 20]",
+        "bar": f"||C - C_fp64||_F / (||A||_F ||B||_F) <= 4 k 2^-53 = {bar:.3e}",
+        "slices": per_s,
+        "fastest_meeting_bar": {"slices": int(best), **per_s[best],
+                                "pct_of_fp64_peak": round(100 * per_s[best]["fp64_equiv_tflops"] /
+                                                          FP64_PEAK_TFLOPS, 1)} if best else None,
+        "kernel": "oz_gemm_kernel (v_mfma_i32_32x32x32_i8), one launch per slice diagonal",
+    }
+
+
+def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
+    """A X = B with A = U diag(s) V^T (U, V Householder-QR factors of Gaussian
+    matrices, s log-spaced to kappa), B Gaussian: plain FP64 LU + solve against
+    LU with FP32 / BF16 trailing updates and FP64 refinement, all to a normwise
+    backward error <= 1e-14.  Times are wall clock around the synchronous
+    device entry (factorisation, solves, residuals, norms), median of 2."""
+    import ctypes as C
+    import torch
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(4401)
+    U, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    V, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    B = torch.randn(n, nrhs, dtype=torch.float64, device=dev, generator=g)
+    sp = torch.cuda.current_stream().cuda_stream
+    out = {"config": f"n={n}, {nrhs} right-hand sides, tol 1e-14", "kappa": {}}
+    for kappa in (1e2, 1e6, 1e10):
+        s = torch.logspace(0, -float(np.log10(kappa)), n, dtype=torch.float64, device=dev)
+        A = (U * s) @ V.t()
+        torch.cuda.synchronize()
+        res = {}
+        for mode, name in ((0, "fp64"), (1, "fp32_ir"), (2, "bf16_ir")):
+            X = torch.zeros(n, nrhs, dtype=torch.float64, device=dev)
+            it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
+            ts = []
+            for _ in range(2):
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                rc = lib.mrm_dev_lineq_solve_ir(mode, A.data_ptr(), n * 8, n, B.data_ptr(), nrhs * 8, X.data_ptr(),
+                                                nrhs * 8, nrhs, 50, 1e-14, C.byref(it), C.byref(be), C.byref(fb), sp)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t)
+            res[name] = {"rc": rc, "ms": round(1e3 * statistics.median(ts), 1), "refinement_steps": it.value,
+                         "backward_error": be.value, "fell_back_to_fp64": bool(fb.value)}
+        for name in ("fp32_ir", "bf16_ir"):
+            res[name]["speedup_vs_fp64"] = round(res["fp64"]["ms"] / res[name]["ms"], 3)
+        out["kappa"][f"{kappa:g}"] = res
+        del A
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dgemm_nn_8192", choices=sorted(WORKLOADS) + ["copy_1gib"])
+    ap.add_argument("--workload", default=None,
+                    choices=sorted(WORKLOADS) + ["ozaki_8192", "lu_ir_16384", "copy_1gib"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-rows", action="store_true", help="headline GEMM only")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -111,89 +285,60 @@ def main():
     from mrmath_amd import _lib
 
     lib = _lib.load()
+    torch.cuda.set_device(0)
     if args.workload == "copy_1gib":
         return copy_calibration(args)
-    trans, M, N, K = WORKLOADS[args.workload]
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    g = torch.Generator(device=dev).manual_seed(8101)
-    rowsA, colsA = (K, M) if trans == 1 else (M, K)
-    rowsB, colsB = (N, K) if trans == 2 else (K, N)
-    A = torch.rand(rowsA, colsA, dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    B = torch.rand(rowsB, colsB, dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    C = torch.empty(M, N, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
+    if args.workload == "ozaki_8192":
+        print(json.dumps({"workload": "ozaki_8192", **ozaki_row(lib)}), flush=True)
+        return
+    if args.workload == "lu_ir_16384":
+        print(json.dumps({"workload": "lu_ir_16384", **lu_ir_row(lib)}), flush=True)
+        return
 
-    def step():
-        rc = lib.mrm_dev_dgemm(trans, M, N, K, A.data_ptr(), colsA * 8, B.data_ptr(), colsB * 8,
-                               C.data_ptr(), N * 8, 0, sp)
-        if rc:
-            _lib.check(rc)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    # per-launch HIP events on the stream the kernel runs on
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s, e in ev:
-        s.record(stream)
-        step()
-        e.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [s.elapsed_time(e) for s, e in ev]
-
-    # sanity: C*x == A*(B*x) to the normwise bound (outside the timed region)
-    x = torch.randn(N, dtype=torch.float64, device=dev, generator=g)
-    Am = A.t() if trans == 1 else A
-    Bm = B.t() if trans == 2 else B
-    lhs = C @ x
-    rhs = Am @ (Bm @ x)
-    rel = (torch.linalg.vector_norm(lhs - rhs) /
-           (torch.linalg.vector_norm(A) * torch.linalg.vector_norm(B) * torch.linalg.vector_norm(x))).item()
-    if not rel <= 4 * K * 2.0 ** -53:
-        print(json.dumps({"error": "GEMM result failed the linearity check", "rel": rel}), file=sys.stderr)
-        sys.exit(1)
-
-    flops = 2.0 * M * N * K
-    ms_per_step = wall * 1e3 / args.steps
-    avg_kernel_ms = statistics.mean(kernel_ms)
-    value = flops / (wall / args.steps) / 1e12
-    achieved = flops / (avg_kernel_ms * 1e-3) / 1e12
-
+    workload = args.workload or "dgemm_nn_8192"
+    r = gemm_run(lib, workload, args.steps, args.warmup)
+    M, N, K = r["M"], r["N"], r["K"]
     out = {
         "metric": "GEMM TFLOP/s (% peak) at FP64 accuracy",
-        "value": round(value, 3),
+        "value": round(r["tflops"], 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (uniform [-1,1) float64, device-resident)",
-        "config": {"workload": args.workload, "M": M, "N": N, "K": K,
-                   "op": ["NN", "TN (MultT1)", "NT (MultT2)"][trans],
+        "config": {"workload": workload, "M": M, "N": N, "K": K, "op": r["op"],
                    "path": "mrm_dev_dgemm -> dgemm_mfma_kernel (v_mfma_f64_4x4x4_4b_f64)"},
-        "pct_of_peak": round(100.0 * value / FP64_PEAK_TFLOPS, 2),
+        "pct_of_peak": round(100.0 * r["tflops"] / FP64_PEAK_TFLOPS, 2),
         "roofline": {
             "bound": "mfma",
-            "achieved": round(achieved, 3),
+            "achieved": round(r["achieved_tflops"], 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": _traffic_from_profile(args.workload),
+            "frac": round(r["achieved_tflops"] / FP64_PEAK_TFLOPS, 4),
+            "traffic": _traffic_from_profile(workload),
             "kernel": "dgemm_mfma_kernel",
-            "avg_kernel_ms": round(avg_kernel_ms, 4),
+            "avg_kernel_ms": round(r["avg_launch_ms"], 4),
         },
-        "check": {"linearity_rel_err": rel, "bound": 4 * K * 2.0 ** -53},
+        "check": {"linearity_rel_err": r["linearity_rel_err"], "bound": 4 * K * 2.0 ** -53},
     }
+    if args.workload is None and not args.no_rows:
+        rows = {}
+        for w in ("dgemm_tn_4096", "dgemm_nt_4096", "syrk_200000x512"):
+            q = gemm_run(lib, w, 10, 2)
+            rows[w] = {"tflops": round(q["tflops"], 3), "ms": round(q["ms_per_step"], 4),
+                       "pct_of_fp64_peak": round(100 * q["tflops"] / FP64_PEAK_TFLOPS, 2),
+                       "linearity_rel_err": q["linearity_rel_err"]}
+            if w.startswith("syrk"):
+                rows[w]["hbm_gbps_algorithmic"] = round((q["M"] * q["K"] + q["M"] * q["N"]) * 8 /
+                                                        (q["ms_per_step"] * 1e-3) / 1e9, 1)
+        rows["ozaki_8192"] = ozaki_row(lib)
+        rows["lu_ir_16384"] = lu_ir_row(lib)
+        out["rows"] = rows
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1, 64)
         out["cpu_baseline"] = cpu_baseline(threads)

@@ -202,8 +202,17 @@ template <bool TA, bool TB, bool VEC>
 __global__ void __launch_bounds__(NTHREADS, 2)
 dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A, int64_t lda,
                   const double* __restrict__ B, int64_t ldb, double* __restrict__ C, int64_t ldc,
-                  int op, int tiles_m, int tiles_n) {
+                  int op, int tiles_m, int tiles_n, int64_t kchunk, int64_t cstride) {
     __shared__ __attribute__((aligned(16))) double smem[2 * 2 * PANEL];
+    if (gridDim.y > 1) {
+        // split K: chunk blockIdx.y of kchunk (a multiple of BK) writes its
+        // partial product to its own C slice
+        const int64_t k0 = int64_t(blockIdx.y) * kchunk;
+        A += TA ? k0 * lda : k0;
+        B += TB ? k0 : k0 * ldb;
+        K = K - k0 < kchunk ? K - k0 : kchunk;
+        C += int64_t(blockIdx.y) * cstride;
+    }
     constexpr bool AKC = !TA;   // A panel rows are M: K-contiguous unless A is stored transposed
     constexpr bool BKC = TB;    // B panel rows are N: K-contiguous only when B is stored transposed
 
@@ -336,14 +345,39 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
 
 template <bool TA, bool TB>
 void launch_variant(bool vec, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
-                    int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, hipStream_t s) {
-    dim3 grid(unsigned(tiles_m) * unsigned(tiles_n)), block(NTHREADS);
+                    int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, hipStream_t s,
+                    int splits = 1, int64_t kchunk = 0, int64_t cstride = 0) {
+    dim3 grid(unsigned(tiles_m) * unsigned(tiles_n), unsigned(splits)), block(NTHREADS);
     if (vec)
         hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n);
+                           op, tiles_m, tiles_n, kchunk, cstride);
     else
         hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n);
+                           op, tiles_m, tiles_n, kchunk, cstride);
+}
+
+// C (op)= sum_c W[c] over `splits` partial products, in chunk order.
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const double* __restrict__ W, int64_t cstride, int splits,
+                                                            double* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
+                                                            int op) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= M * N) return;
+    const int64_t i = idx / N, j = idx % N;
+    double acc = W[idx];
+    for (int c = 1; c < splits; ++c) acc += W[int64_t(c) * cstride + idx];
+    double* cp = C + i * ldc + j;
+    if (op == MRM_DO_NONE) *cp = acc;
+    else if (op == MRM_DO_ADD) *cp = *cp + acc;
+    else *cp = *cp - acc;
+}
+
+// Split-K when the output has too few tiles to fill the GPU and K is long
+// (the covariance shape X^T X with X tall): up to 64 chunks of at least 512.
+int splitk_count(int64_t tiles, int64_t K) {
+    if (tiles >= 128 || K < 4096) return 1;
+    int64_t p = (512 + tiles - 1) / tiles;
+    p = p < K / 512 ? p : K / 512;
+    return int(p < 64 ? p : 64);
 }
 
 // C (M x N) = 0: the empty product for doNone.
@@ -385,6 +419,25 @@ int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A
     }
     const bool vec = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 &&
                      ((lda | ldb) & 15) == 0;
+    const int splits = splitk_count(tm * tn, K);
+    if (splits > 1) {
+        const int64_t kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
+        const int used = int((K + kchunk - 1) / kchunk);
+        const int64_t cstride = M * N;
+        void* W = nullptr;
+        MRM_HIP_TRY(hipMallocAsync(&W, size_t(used) * size_t(cstride) * 8, stream));
+        double* w = static_cast<double*>(W);
+        switch (trans) {
+            case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), stream, used, kchunk, cstride); break;
+            case MRM_TN: launch_variant<true, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), stream, used, kchunk, cstride); break;
+            default:     launch_variant<false, true>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), stream, used, kchunk, cstride); break;
+        }
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3(unsigned((M * N + 255) / 256)), dim3(256), 0, stream, w, cstride,
+                           used, C, ec, M, N, op);
+        MRM_HIP_TRY(hipGetLastError());
+        MRM_HIP_TRY(hipFreeAsync(W, stream));
+        return MRM_OK;
+    }
     switch (trans) {
         case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), stream); break;
         case MRM_TN: launch_variant<true, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), stream); break;

@@ -237,3 +237,34 @@ def test_device_entry_t1_t2_4096_sampled(trans):
     rows = [0, 17, 2048, 4095]
     ref = orc.generic_mult(np.ascontiguousarray(Am[rows]), np.ascontiguousarray(Bm))
     assert orc.gemm_err(np.ascontiguousarray(Cn[rows]), ref, np.ascontiguousarray(Am[rows]), np.ascontiguousarray(Bm)) <= tol(n)
+
+
+def test_covariance_shape_split_k():
+    """X^T X with X tall (the covariance shape of Statistics.Covariance,
+    src/Statistics.pas:599-639): few output tiles, long K, so the GEMM splits K
+    and sums the chunks in a fixed order.  Checked against the oracle's
+    GenericTranspMtxMult on a 20000 x 96 case and, at 200000 x 512, on sampled
+    rows plus symmetry."""
+    import torch
+    lib = _lib.load()
+    X = orc.pm1_matrix(96, 20000, 2001)
+    C = np.zeros((96, 96))
+    ss.MatrixMultT1(C, X, X)
+    ref = orc.generic_transp_mult(X, X)
+    assert orc.gemm_err(C, ref, np.ascontiguousarray(X.T), X) <= 4 * 20000 * U
+    # full size on device
+    n, k = 512, 200000
+    g = torch.Generator(device="cuda").manual_seed(2001)
+    Xd = torch.rand(k, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    Cd = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    s_ = torch.cuda.current_stream().cuda_stream
+    assert lib.mrm_dev_dgemm(_lib.TN, n, n, k, Xd.data_ptr(), n * 8, Xd.data_ptr(), n * 8, Cd.data_ptr(), n * 8, 0,
+                             s_) == 0
+    torch.cuda.synchronize()
+    Cn = Cd.cpu().numpy()
+    Xn = Xd.cpu().numpy()
+    assert np.max(np.abs(Cn - Cn.T)) <= 4 * k * U * np.max(np.abs(np.diag(Cn)))
+    cols = np.array([0, 1, 255, 511])
+    ref_rows = orc.generic_transp_mult(np.ascontiguousarray(Xn[:, cols]), Xn)
+    err = np.linalg.norm(Cn[cols] - ref_rows) / (np.linalg.norm(Xn[:, cols]) * np.linalg.norm(Xn))
+    assert err <= 4 * k * U, err

@@ -206,6 +206,52 @@ MRM_API int32_t mrm_lineq_solve_ex(const double* A, int64_t LineWidthA, int64_t 
                                    int64_t LineWidthB, double* X, int64_t LineWidthX, int64_t width2,
                                    int64_t NumRefinments, int32_t mode);
 
+/* ---- batched small problems (persistent launches replayed from a hipGraph) */
+
+/* One GEMM C = A * B (row-major, device pointers, strides in bytes). */
+typedef struct mrm_gemm_desc {
+    int64_t M, N, K;
+    const double* A;
+    int64_t lda;
+    const double* B;
+    int64_t ldb;
+    double* C;
+    int64_t ldc;
+} mrm_gemm_desc;
+
+/* One system A x = b, n <= 128 (device pointers; A and b are left unchanged). */
+typedef struct mrm_lu_desc {
+    int64_t n;
+    const double* A;
+    int64_t lda;
+    const double* b;
+    double* x;
+} mrm_lu_desc;
+
+/* Plans `count` FP64 GEMMs of any sizes (each M, N <= 2^20) as one persistent
+ * grouped launch over T x T output tiles, T in {32, 64, 128}: tile = 0 lets the
+ * library time the batch with each T and keep the fastest (this runs the
+ * batch); the launch is captured into a hipGraph.  The descriptors are copied;
+ * the matrices must stay allocated while the plan is used.  The callers
+ * replaced are loops of MatrixMult over small matrices (the reference's
+ * small-size path FMAMatrixMult, src/FMAMatrixOperations.pas:94-143). */
+MRM_API int32_t mrm_batched_dgemm_plan(const mrm_gemm_desc* problems, int64_t count, int32_t tile, void** plan);
+
+/* Plans `count` linear solves A_i x_i = b_i (n_i <= 128, one right-hand side)
+ * with partial pivoting, grouped by size class into persistent launches
+ * captured in a hipGraph; mode MRM_SOLVE_FP64, or MRM_SOLVE_FP32_IR (FP32
+ * factors, two FP64 refinement steps against A_i).  status (device, count
+ * int32, may be NULL) receives MRM_OK or MRM_SINGULAR per system at each run.
+ * The per-system MatrixLinEQSolve of the reference (src/LinAlgLU.pas:481-592). */
+MRM_API int32_t mrm_batched_lu_plan(const mrm_lu_desc* systems, int64_t count, int32_t mode, int32_t* status,
+                                    void** plan);
+
+/* Replays a plan on `stream` (one hipGraphLaunch); asynchronous. */
+MRM_API int32_t mrm_batched_run(void* plan, void* stream);
+/* The GEMM tile size a plan uses (64 threads per system for solve plans). */
+MRM_API int32_t mrm_batched_plan_tile(void* plan);
+MRM_API void mrm_batched_destroy(void* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,11 @@ SIGNATURES = {
     "mrm_dev_lineq_solve_ir": (_i32, [_i32, _pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i32, C.c_double, _pv, _pv,
                                       _pv, _pv]),
     "mrm_lineq_solve_ex": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64, _i32]),
+    "mrm_batched_dgemm_plan": (_i32, [_pv, _i64, _i32, _pv]),
+    "mrm_batched_lu_plan": (_i32, [_pv, _i64, _i32, _pv, _pv]),
+    "mrm_batched_run": (_i32, [_pv, _pv]),
+    "mrm_batched_plan_tile": (_i32, [_pv]),
+    "mrm_batched_destroy": (None, [_pv]),
     "mrm_lu_decomp_inplace": (_i32, [_pd, _i64, _i64, _pv]),
     "mrm_lu_backsubst": (_i32, [_pd, _i64, _i64, _pv, _pd, _i64]),
     "mrm_lineq_solve": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64]),

@@ -1,0 +1,581 @@
+// mrm_batched.hip -- thousands of small, uneven FP64 GEMMs and small linear
+// solves in persistent launches replayed from a hipGraph.
+//
+// The reference has no batched entry: its callers loop over MatrixMult /
+// MatrixLinEQSolve per problem (src/MatrixASMStubSwitch.pas:769-776,
+// src/LinAlgLU.pas:481-592), and the small-size path of its multiply is
+// FMAMatrixMult (src/FMAMatrixOperations.pas:94-143).  Here a plan gathers the
+// problem list once:
+//   GEMM: every problem is cut into T x T output tiles, all tiles of all
+//     problems form one list, and a persistent grid of workgroups takes tiles
+//     from an atomic counter (largest K first).  T in {32, 64, 128} is picked
+//     by timing the actual batch (the autotuner) unless the caller fixes it.
+//   LU solve: systems are grouped by size class (n <= 32, 64, 128); per class
+//     one persistent launch where each wave takes whole systems from a counter,
+//     holds A in LDS, factors it with partial pivoting (the reference's rule:
+//     first row of maximal |a|, singular at |pivot| <= 10/MaxDouble) and
+//     substitutes.  FP64, or FP32 factors with FP64 residuals and refinement
+//     (x += (LU)^-1 (b - A x), A read again in FP64 from memory).
+// The counter reset and the launches are captured once into a hipGraph; a run
+// is one hipGraphLaunch.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+#include <vector>
+
+#include "mrm_common.h"
+#include "mrm_host.h"
+
+namespace {
+
+constexpr int BG_THREADS = 256;
+constexpr int BG_BK = 16;
+
+struct GemmP {  // element strides
+    int64_t M, N, K, lda, ldb, ldc;
+    const double* A;
+    const double* B;
+    double* C;
+};
+
+struct Tile {
+    int p, m0, n0, pad;
+};
+
+__global__ void zero_counters_kernel(unsigned* c, int n) {
+    if (int(threadIdx.x) < n) c[threadIdx.x] = 0;
+}
+
+// One T x T tile of C = A * B per iteration of the persistent loop.  Four
+// waves in 2 x 2, each (T/2) x (T/2) as (T/32)^2 blocks of 16 x 16 on
+// v_mfma_f64_16x16x4f64 (lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// result register v of lane l is row l/16 + 4v, column l%16).  K in chunks of
+// 16 staged in LDS as [k][row], the next chunk prefetched into registers.
+template <int T>
+__global__ void __launch_bounds__(BG_THREADS) bg_gemm_kernel(const GemmP* __restrict__ probs,
+                                                             const Tile* __restrict__ tiles, int ntiles,
+                                                             unsigned* __restrict__ counter) {
+    constexpr int BP = T / 32;               // 16x16 blocks per wave per dimension
+    constexpr int EA = T * BG_BK / BG_THREADS;  // elements per thread per chunk
+    constexpr int LDT = T + 2;
+    __shared__ double As[BG_BK][LDT];
+    __shared__ double Bs[BG_BK][LDT];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    for (;;) {
+        if (tid == 0) s_item = int(atomicAdd(counter, 1u));
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= ntiles) return;
+        const Tile tl = tiles[item];
+        const GemmP pr = probs[tl.p];
+        const int64_t m0 = tl.m0, n0 = tl.n0;
+        double ra[EA], rb[EA];
+        auto fetch = [&](int64_t k0) {
+#pragma unroll
+            for (int i = 0; i < EA; ++i) {
+                const int idx = tid + BG_THREADS * i;
+                const int k = idx % BG_BK, r = idx / BG_BK;  // A: k fastest (row-major A)
+                const int64_t gr = m0 + r, gk = k0 + k;
+                ra[i] = (gr < pr.M && gk < pr.K) ? pr.A[gr * pr.lda + gk] : 0.0;
+                const int c = idx % T, kb = idx / T;        // B: column fastest
+                const int64_t gc = n0 + c, gkb = k0 + kb;
+                rb[i] = (gc < pr.N && gkb < pr.K) ? pr.B[gkb * pr.ldb + gc] : 0.0;
+            }
+        };
+        auto stash = [&]() {
+#pragma unroll
+            for (int i = 0; i < EA; ++i) {
+                const int idx = tid + BG_THREADS * i;
+                As[idx % BG_BK][idx / BG_BK] = ra[i];
+                Bs[idx / T][idx % T] = rb[i];
+            }
+        };
+        mrm_d4 acc[BP][BP];
+#pragma unroll
+        for (int p = 0; p < BP; ++p)
+#pragma unroll
+            for (int c = 0; c < BP; ++c) acc[p][c] = mrm_d4{0.0, 0.0, 0.0, 0.0};
+        fetch(0);
+        for (int64_t k0 = 0; k0 < pr.K; k0 += BG_BK) {
+            stash();
+            __syncthreads();
+            if (k0 + BG_BK < pr.K) fetch(k0 + BG_BK);
+#pragma unroll
+            for (int kk = 0; kk < BG_BK / 4; ++kk) {
+                const int k = 4 * kk + (lane >> 4);
+                double a[BP], b[BP];
+#pragma unroll
+                for (int p = 0; p < BP; ++p) a[p] = As[k][wm * (T / 2) + 16 * p + (lane & 15)];
+#pragma unroll
+                for (int c = 0; c < BP; ++c) b[c] = Bs[k][wn * (T / 2) + 16 * c + (lane & 15)];
+#pragma unroll
+                for (int p = 0; p < BP; ++p)
+#pragma unroll
+                    for (int c = 0; c < BP; ++c)
+                        acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p], b[c], acc[p][c], 0, 0, 0);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int p = 0; p < BP; ++p)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = m0 + wm * (T / 2) + 16 * p + (lane >> 4) + 4 * v;
+                if (row >= pr.M) continue;
+#pragma unroll
+                for (int c = 0; c < BP; ++c) {
+                    const int64_t col = n0 + wn * (T / 2) + 16 * c + (lane & 15);
+                    if (col < pr.N) pr.C[row * pr.ldc + col] = acc[p][c][v];
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched small LU solves: one wave per system, A in LDS (row stride n+1).
+// ---------------------------------------------------------------------------
+struct LuP {  // element stride
+    int64_t n, lda;
+    const double* A;
+    const double* b;
+    double* x;
+    int idx;  // position in the caller's list (status output)
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <typename T>
+__device__ __forceinline__ T tiny_pivot();
+template <>
+__device__ __forceinline__ double tiny_pivot<double>() { return 10.0 / DBL_MAX; }
+template <>
+__device__ __forceinline__ float tiny_pivot<float>() { return 10.0f / FLT_MAX; }
+
+// LU with partial pivoting of the n x n matrix L (row stride ld) in LDS by one
+// wave; rows are exchanged physically, piv[j] records the row exchanged with j.
+// Returns false when singular.
+template <typename T>
+__device__ bool wave_lu(T* L, int ld, int n, int* piv, int lane) {
+    bool ok = true;
+    for (int j = 0; j < n; ++j) {
+        // first row of maximal |a| in column j
+        T best = T(-1);
+        int brow = n;
+        for (int i = j + lane; i < n; i += 64) {
+            const T v = fabs(L[i * ld + j]);
+            if (v > best) { best = v; brow = i; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(best, off, 64);
+            const int orow = __shfl_xor(brow, off, 64);
+            if (ov > best || (ov == best && orow < brow)) { best = ov; brow = orow; }
+        }
+        if (lane == 0) piv[j] = brow;
+        if (!(best > tiny_pivot<T>())) { ok = false; break; }
+        if (brow != j)
+            for (int k = lane; k < n; k += 64) {
+                const T t = L[j * ld + k];
+                L[j * ld + k] = L[brow * ld + k];
+                L[brow * ld + k] = t;
+            }
+        wave_sync();
+        const T rinv = T(1) / L[j * ld + j];
+        for (int i = j + 1 + lane; i < n; i += 64) L[i * ld + j] *= rinv;
+        wave_sync();
+        const int w = n - j - 1;
+        for (int q = lane; q < w * w; q += 64) {
+            const int i = j + 1 + q / w, k = j + 1 + q % w;
+            L[i * ld + k] -= L[i * ld + j] * L[j * ld + k];
+        }
+        wave_sync();
+    }
+    wave_sync();
+    return ok;
+}
+
+// y := (LU)^-1 P y in place (y in LDS, type Y), with the factors of wave_lu.
+template <typename T, typename Y>
+__device__ void wave_lu_subst(const T* L, int ld, int n, const int* piv, Y* y, int lane) {
+    if (lane == 0)
+        for (int j = 0; j < n; ++j) {
+            const int r = piv[j];
+            if (r != j) { const Y t = y[j]; y[j] = y[r]; y[r] = t; }
+        }
+    wave_sync();
+    for (int k = 0; k < n; ++k) {            // unit lower
+        const Y yk = y[k];
+        for (int i = k + 1 + lane; i < n; i += 64) y[i] -= Y(L[i * ld + k]) * yk;
+        wave_sync();
+    }
+    for (int k = n - 1; k >= 0; --k) {       // upper
+        if (lane == 0) y[k] = y[k] / Y(L[k * ld + k]);
+        wave_sync();
+        const Y yk = y[k];
+        for (int i = lane; i < k; i += 64) y[i] -= Y(L[i * ld + k]) * yk;
+        wave_sync();
+    }
+}
+
+// MODE 0: FP64 factors.  MODE 1: FP32 factors, `refine` FP64 refinement steps.
+// Dynamic LDS per wave: nmax*(nmax+1) factors of type T, 2*nmax doubles, nmax ints.
+template <int MODE>
+__global__ void __launch_bounds__(64) bl_solve_kernel(const LuP* __restrict__ sys, int count, int nmax,
+                                                      unsigned* __restrict__ counter, int* __restrict__ status,
+                                                      int refine) {
+    using T = typename std::conditional<MODE == 0, double, float>::type;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int lane = threadIdx.x;
+    T* L = reinterpret_cast<T*>(lds);
+    double* xv = reinterpret_cast<double*>(lds + ((size_t(nmax) * (nmax + 1) * sizeof(T) + 15) & ~size_t(15)));
+    double* rv = xv + nmax;
+    int* piv = reinterpret_cast<int*>(rv + nmax);
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = int(atomicAdd(counter, 1u));
+        item = __shfl(item, 0, 64);
+        if (item >= count) return;
+        const LuP s = sys[item];
+        const int n = int(s.n), ld = n + 1;
+        for (int q = lane; q < n * n; q += 64) L[(q / n) * ld + q % n] = T(s.A[(q / n) * s.lda + q % n]);
+        for (int i = lane; i < n; i += 64) xv[i] = s.b[i];
+        wave_sync();
+        const bool ok = wave_lu<T>(L, ld, n, piv, lane);
+        if (ok) {
+            if (MODE == 0) {
+                wave_lu_subst<T, double>(L, ld, n, piv, xv, lane);
+            } else {
+                // FP32 solve of the FP64 right-hand side, then FP64 refinement
+                float* yv = reinterpret_cast<float*>(rv);
+                for (int i = lane; i < n; i += 64) yv[i] = float(xv[i]);
+                wave_sync();
+                wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
+                for (int i = lane; i < n; i += 64) xv[i] = double(yv[i]);
+                wave_sync();
+                for (int it = 0; it < refine; ++it) {
+                    // r = b - A x with A and b from memory in FP64 (row i per lane)
+                    for (int i = lane; i < n; i += 64) {
+                        double r = s.b[i];
+                        const double* arow = s.A + int64_t(i) * s.lda;
+                        for (int k = 0; k < n; ++k) r -= arow[k] * xv[k];
+                        yv[i] = float(r);  // lanes own disjoint rows
+                    }
+                    wave_sync();
+                    wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
+                    for (int i = lane; i < n; i += 64) xv[i] += double(yv[i]);
+                    wave_sync();
+                }
+            }
+            for (int i = lane; i < n; i += 64) s.x[i] = xv[i];
+        }
+        if (lane == 0 && status) status[s.idx] = ok ? MRM_OK : MRM_SINGULAR;
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+struct Launch {
+    const void* fn;
+    int grid;
+    int block;
+    size_t lds;
+    std::vector<void*> args;
+};
+
+struct Plan {
+    int kind = 0;          // 1 GEMM, 2 LU
+    int tile = 0;          // GEMM tile chosen
+    int mode = 0;          // LU precision
+    void* dev = nullptr;   // descriptors, tiles, counters (one allocation)
+    unsigned* counters = nullptr;
+    int ncounters = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // argument storage referenced by the captured launches
+    GemmP* probs = nullptr;
+    Tile* tiles = nullptr;
+    int ntiles = 0;
+    LuP* sys[3] = {nullptr, nullptr, nullptr};
+    int counts[3] = {0, 0, 0};
+    int nmax[3] = {32, 64, 128};
+    int* status = nullptr;
+    int refine = 2;
+};
+
+int resident_grid(const void* fn, int block, size_t lds) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, lds) != hipSuccess || per < 1) per = 1;
+    return cus * per;
+}
+
+template <int T>
+const void* gemm_fn() {
+    return reinterpret_cast<const void*>(&bg_gemm_kernel<T>);
+}
+
+const void* gemm_fn_for(int t) {
+    return t == 32 ? gemm_fn<32>() : t == 64 ? gemm_fn<64>() : gemm_fn<128>();
+}
+
+// Tile list for tile size t, problems with the longest K first.
+std::vector<Tile> make_tiles(const std::vector<GemmP>& probs, int t) {
+    std::vector<int> order(probs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return probs[a].K > probs[b].K; });
+    std::vector<Tile> tiles;
+    for (int p : order) {
+        if (probs[p].M == 0 || probs[p].N == 0) continue;
+        for (int64_t m = 0; m < probs[p].M; m += t)
+            for (int64_t n = 0; n < probs[p].N; n += t) tiles.push_back(Tile{p, int(m), int(n), 0});
+    }
+    return tiles;
+}
+
+int launch_gemm(Plan& pl, int t, hipStream_t s) {
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, pl.counters, 1);
+    const void* fn = gemm_fn_for(t);
+    const int grid = resident_grid(fn, BG_THREADS, 0);
+    void* args[] = {&pl.probs, &pl.tiles, &pl.ntiles, &pl.counters};
+    MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(BG_THREADS), args, 0, s));
+    return MRM_OK;
+}
+
+size_t lu_lds(int nmax, int mode) {
+    const size_t fac = size_t(nmax) * (nmax + 1) * (mode == 0 ? 8 : 4);
+    return ((fac + 15) & ~size_t(15)) + size_t(nmax) * 16 + size_t(nmax) * 4;
+}
+
+int launch_lu(Plan& pl, hipStream_t s) {
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, pl.counters, 3);
+    for (int c = 0; c < 3; ++c) {
+        if (pl.counts[c] == 0) continue;
+        const void* fn = pl.mode == 0 ? reinterpret_cast<const void*>(&bl_solve_kernel<0>)
+                                      : reinterpret_cast<const void*>(&bl_solve_kernel<1>);
+        const size_t lds = lu_lds(pl.nmax[c], pl.mode);
+        const int grid = resident_grid(fn, 64, lds);
+        unsigned* ctr = pl.counters + c;
+        void* args[] = {&pl.sys[c], &pl.counts[c], &pl.nmax[c], &ctr, &pl.status, &pl.refine};
+        MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s));
+    }
+    return MRM_OK;
+}
+
+int capture(Plan& pl, hipStream_t s) {
+    MRM_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = pl.kind == 1 ? launch_gemm(pl, pl.tile, s) : launch_lu(pl, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    MRM_HIP_TRY(e);
+    pl.graph = g;
+    MRM_HIP_TRY(hipGraphInstantiate(&pl.exec, g, nullptr, nullptr, 0));
+    return MRM_OK;
+}
+
+void destroy(Plan* pl) {
+    if (!pl) return;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->dev) (void)hipFree(pl->dev);
+    delete pl;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" {
+
+int32_t mrm_batched_dgemm_plan(const mrm_gemm_desc* problems, int64_t count, int32_t tile, void** plan) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if (!plan || count < 0 || count > (1 << 24) || (count > 0 && !problems) ||
+        (tile != 0 && tile != 32 && tile != 64 && tile != 128)) {
+        mrm_set_error(MRM_BAD_ARG, "batched gemm: bad plan arguments");
+        return MRM_BAD_ARG;
+    }
+    *plan = nullptr;
+    std::vector<GemmP> probs(static_cast<size_t>(count));
+    for (int64_t i = 0; i < count; ++i) {
+        const mrm_gemm_desc& d = problems[i];
+        if (d.M < 0 || d.N < 0 || d.K < 0 || d.M > (1 << 20) || d.N > (1 << 20) || ((d.lda | d.ldb | d.ldc) & 7) ||
+            (d.M > 0 && d.K > 0 && (d.lda < d.K * 8 || !d.A)) || (d.K > 0 && d.N > 0 && (d.ldb < d.N * 8 || !d.B)) ||
+            (d.M > 0 && d.N > 0 && (d.ldc < d.N * 8 || !d.C))) {
+            mrm_set_error(MRM_BAD_DIMS, "batched gemm: problem %lld has bad sizes, strides or pointers", (long long)i);
+            return MRM_BAD_DIMS;
+        }
+        probs[size_t(i)] = GemmP{d.M, d.N, d.K, d.lda / 8, d.ldb / 8, d.ldc / 8, d.A, d.B, d.C};
+    }
+    Plan* pl = new Plan;
+    pl->kind = 1;
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    // the largest tile list (T = 32) bounds the device storage
+    const std::vector<Tile> t32 = make_tiles(probs, 32);
+    const size_t bytes = align256(probs.size() * sizeof(GemmP) + 1) + align256(t32.size() * sizeof(Tile) + 1) + 256;
+    if (hipMalloc(&pl->dev, bytes) != hipSuccess) {
+        delete pl;
+        mrm_set_error(MRM_NO_MEMORY, "batched gemm: cannot allocate the plan");
+        return MRM_NO_MEMORY;
+    }
+    char* base = static_cast<char*>(pl->dev);
+    pl->probs = reinterpret_cast<GemmP*>(base);
+    pl->tiles = reinterpret_cast<Tile*>(base + align256(probs.size() * sizeof(GemmP) + 1));
+    pl->counters = reinterpret_cast<unsigned*>(base + bytes - 256);
+    pl->ncounters = 1;
+    if (!probs.empty()) rc = hipMemcpyAsync(pl->probs, probs.data(), probs.size() * sizeof(GemmP),
+                                            hipMemcpyHostToDevice, s) == hipSuccess ? MRM_OK : MRM_HIP_ERROR;
+    auto use = [&](int t) -> int {
+        const std::vector<Tile> tl = t == 32 ? t32 : make_tiles(probs, t);
+        pl->ntiles = int(tl.size());
+        if (!tl.empty() && hipMemcpyAsync(pl->tiles, tl.data(), tl.size() * sizeof(Tile), hipMemcpyHostToDevice,
+                                          s) != hipSuccess) {
+            mrm_set_error(MRM_HIP_ERROR, "batched gemm: tile upload failed");
+            return MRM_HIP_ERROR;
+        }
+        return MRM_OK;
+    };
+    int best = tile;
+    if (!rc && tile == 0) {
+        // autotune: time each tile size on the batch itself (C = A*B is
+        // idempotent), best of two runs each
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        float best_ms = 1e30f;
+        for (int t : {32, 64, 128}) {
+            if ((rc = use(t))) break;
+            float tbest = 1e30f;
+            for (int rep = 0; rep < 2 && !rc; ++rep) {
+                (void)hipEventRecord(e0, s);
+                rc = launch_gemm(*pl, t, s);
+                (void)hipEventRecord(e1, s);
+                if (!rc && hipEventSynchronize(e1) == hipSuccess) {
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    tbest = std::min(tbest, ms);
+                }
+            }
+            if (tbest < best_ms) {
+                best_ms = tbest;
+                best = t;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    if (!rc) rc = use(best);
+    pl->tile = best;
+    if (!rc) rc = capture(*pl, s);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MRM_HIP_ERROR;
+    if (rc) {
+        destroy(pl);
+        return rc;
+    }
+    *plan = pl;
+    return MRM_OK;
+}
+
+int32_t mrm_batched_lu_plan(const mrm_lu_desc* systems, int64_t count, int32_t mode, int32_t* status, void** plan) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if (!plan || count < 0 || count > (1 << 24) || (count > 0 && !systems) ||
+        (mode != MRM_SOLVE_FP64 && mode != MRM_SOLVE_FP32_IR)) {
+        mrm_set_error(MRM_BAD_ARG, "batched solve: bad plan arguments (modes: FP64, FP32_IR)");
+        return MRM_BAD_ARG;
+    }
+    *plan = nullptr;
+    std::vector<LuP> cls[3];
+    for (int64_t i = 0; i < count; ++i) {
+        const mrm_lu_desc& d = systems[i];
+        if (d.n < 0 || d.n > 128 || (d.lda & 7) || (d.n > 0 && (d.lda < d.n * 8 || !d.A || !d.b || !d.x))) {
+            mrm_set_error(MRM_BAD_DIMS, "batched solve: system %lld has n outside 0..128 or a bad layout",
+                          (long long)i);
+            return MRM_BAD_DIMS;
+        }
+        const LuP p{d.n, d.lda / 8, d.A, d.b, d.x, int(i)};
+        cls[d.n <= 32 ? 0 : d.n <= 64 ? 1 : 2].push_back(p);
+    }
+    Plan* pl = new Plan;
+    pl->kind = 2;
+    pl->mode = mode;
+    pl->status = status;
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    size_t bytes = 256;
+    for (auto& c : cls) {
+        // largest systems first inside a class
+        std::stable_sort(c.begin(), c.end(), [](const LuP& a, const LuP& b) { return a.n > b.n; });
+        bytes += align256(c.size() * sizeof(LuP) + 1);
+    }
+    if (hipMalloc(&pl->dev, bytes) != hipSuccess) {
+        delete pl;
+        mrm_set_error(MRM_NO_MEMORY, "batched solve: cannot allocate the plan");
+        return MRM_NO_MEMORY;
+    }
+    char* base = static_cast<char*>(pl->dev);
+    pl->counters = reinterpret_cast<unsigned*>(base);
+    pl->ncounters = 3;
+    size_t off = 256;
+    for (int c = 0; c < 3; ++c) {
+        pl->sys[c] = reinterpret_cast<LuP*>(base + off);
+        pl->counts[c] = int(cls[c].size());
+        if (!cls[c].empty() &&
+            hipMemcpyAsync(pl->sys[c], cls[c].data(), cls[c].size() * sizeof(LuP), hipMemcpyHostToDevice, s) !=
+                hipSuccess)
+            rc = MRM_HIP_ERROR;
+        off += align256(cls[c].size() * sizeof(LuP) + 1);
+    }
+    for (int m : {0, 1}) {
+        const void* fn = m == 0 ? reinterpret_cast<const void*>(&bl_solve_kernel<0>)
+                                : reinterpret_cast<const void*>(&bl_solve_kernel<1>);
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lu_lds(128, m))) != hipSuccess)
+            rc = MRM_HIP_ERROR;
+    }
+    pl->tile = 64;
+    if (!rc) rc = capture(*pl, s);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MRM_HIP_ERROR;
+    if (rc) {
+        if (rc == MRM_HIP_ERROR) mrm_set_error(MRM_HIP_ERROR, "batched solve: plan set-up failed");
+        destroy(pl);
+        return rc;
+    }
+    *plan = pl;
+    return MRM_OK;
+}
+
+int32_t mrm_batched_run(void* plan, void* stream) {
+    if (!plan) {
+        mrm_set_error(MRM_BAD_ARG, "null plan");
+        return MRM_BAD_ARG;
+    }
+    Plan* pl = static_cast<Plan*>(plan);
+    MRM_HIP_TRY(hipGraphLaunch(pl->exec, mrm_resolve_stream(stream)));
+    return MRM_OK;
+}
+
+int32_t mrm_batched_plan_tile(void* plan) { return plan ? static_cast<Plan*>(plan)->tile : 0; }
+
+void mrm_batched_destroy(void* plan) {
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    destroy(static_cast<Plan*>(plan));
+}
+
+}  // extern "C"

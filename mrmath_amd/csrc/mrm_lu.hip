@@ -422,15 +422,6 @@ trsm_upper_block_kernel(const T* __restrict__ U, int64_t ldu, T* __restrict__ X,
 }
 
 // Y[i][c] = B[perm[i]][c]
-template <typename T>
-__global__ void gather_rows_kernel(const T* __restrict__ B, int64_t ldb, T* __restrict__ Y, int64_t ldy,
-                                   const int* __restrict__ perm, int n, int ncols) {
-    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= int64_t(n) * ncols) return;
-    const int i = int(idx / ncols), c = int(idx % ncols);
-    Y[int64_t(i) * ldy + c] = B[int64_t(perm[i]) * ldb + c];
-}
-
 // X += D (rows x cols, element strides)
 template <typename T>
 __global__ void add_inplace_kernel(T* __restrict__ X, int64_t ldx, const T* __restrict__ D, int64_t ldd, int rows,
@@ -580,21 +571,6 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
     return MRM_OK;
 }
 
-// X (n x nrhs) = (LU)^-1 P B, with P given by perm.
-template <typename T>
-int lu_solve(const T* LU, int64_t lda, int n, const int* perm, const T* B, int64_t ldb, T* X, int64_t ldx, int nrhs,
-             hipStream_t s) {
-    if (n == 0 || nrhs == 0) return MRM_OK;
-    hipLaunchKernelGGL(gather_rows_kernel<T>, dim3(grid1(int64_t(n) * nrhs, 256)), dim3(256), 0, s, B, ldb, X, ldx,
-                       perm, n, nrhs);
-    int rc = trsm_lower_unit<T>(LU, lda, X, ldx, n, nrhs, s);
-    if (rc) return rc;
-    rc = trsm_upper<T>(LU, lda, X, ldx, n, nrhs, s);
-    if (rc) return rc;
-    MRM_HIP_TRY(hipGetLastError());
-    return MRM_OK;
-}
-
 template int lu_factor<double>(double*, int64_t, int, int*, int*, LuWork&, hipStream_t, int);
 
 size_t lu_lowp_bytes(int n, int mode) {
@@ -615,8 +591,6 @@ int lu_lowp_reserve(LuWork& ws, int n, int mode) {
     ws.lowp_bytes = need;
     return MRM_OK;
 }
-template int lu_solve<double>(const double*, int64_t, int, const int*, const double*, int64_t, double*, int64_t, int,
-                              hipStream_t);
 
 template <typename T>
 void add_inplace(T* X, int64_t ldx, const T* D, int64_t ldd, int rows, int cols, hipStream_t s) {

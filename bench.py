@@ -10,6 +10,8 @@ carries the other measured rows of the path under "rows":
   syrk_200000x512                 X^T X with X 200000 x 512 (split-K GEMM)
   ozaki_8192                      FP64 GEMM emulated with INT8 slices, s = 3..7,
                                   wide-range inputs (2 res53 - 1) 2^e, e in [-20, 20]
+  batched                         8192 uneven FP64 GEMMs (m, n, k in [16, 256]) and
+                                  20000 small solves (n in [8, 128]), hipGraph plans
   lu_ir_16384                     A X = B, n = 16384, 64 right-hand sides,
                                   A = U diag(s) V^T with kappa 1e2 / 1e6 / 1e10:
                                   plain FP64 LU vs FP32- and BF16-update LU with
@@ -18,7 +20,7 @@ carries the other measured rows of the path under "rows":
   python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-rows]
 
 --workload runs one workload alone (dgemm_nn_8192 default, dgemm_tn_4096,
-dgemm_nt_4096, syrk_200000x512, ozaki_8192, lu_ir_16384, and copy_1gib: a
+dgemm_nt_4096, syrk_200000x512, ozaki_8192, batched, lu_ir_16384, and copy_1gib: a
 plain 1 GiB device copy, the calibration kernel for the HBM byte counters).
 """
 from __future__ import annotations
@@ -265,13 +267,76 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
     return out
 
 
+def batched_row(lib, reps: int = 5):
+    """8192 FP64 GEMMs with m, n, k uniform in [16, 256] (seed 5101) and 20000
+    solves with n uniform in [8, 128], A Gaussian + n I, one right-hand side
+    (seed 5201), each as one hipGraph replay of its plan (median of `reps`)."""
+    import torch
+    from mrmath_amd import batched as bt
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5101)
+    sizes = [tuple(int(v) for v in rng.integers(16, 257, 3)) for _ in range(8192)]
+    g = torch.Generator(device=dev).manual_seed(5101)
+    probs = []
+    for m, n, k in sizes:
+        probs.append((torch.rand(m, k, dtype=torch.float64, device=dev, generator=g) * 2 - 1,
+                      torch.rand(k, n, dtype=torch.float64, device=dev, generator=g) * 2 - 1,
+                      torch.empty(m, n, dtype=torch.float64, device=dev)))
+    flops = sum(2.0 * m * n * k for m, n, k in sizes)
+    nbytes = sum(8.0 * (m * k + k * n + m * n) for m, n, k in sizes)
+
+    def timed(plan):
+        plan.run()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            plan.run()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return statistics.median(ts)
+
+    gp = bt.dgemm_plan(probs, 0)
+    ms = timed(gp)
+    floor_ms = max(flops / (FP64_PEAK_TFLOPS * 1e12), nbytes / (HBM_PEAK_GBS * 1e9)) * 1e3
+    out = {"gemm_8192": {"ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 2), "tile": gp.tile,
+                         "gbps": round(nbytes / ms / 1e6, 1), "roofline_floor_ms": round(floor_ms, 4),
+                         "frac_of_floor": round(floor_ms / ms, 4)}}
+    gp.close()
+    del probs
+    rng = np.random.default_rng(5201)
+    ns = [int(v) for v in rng.integers(8, 129, 20000)]
+    g = torch.Generator(device=dev).manual_seed(5201)
+    systems = []
+    for n in ns:
+        A = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+        A.diagonal().add_(float(n))
+        systems.append((A, torch.randn(n, dtype=torch.float64, device=dev, generator=g),
+                        torch.empty(n, dtype=torch.float64, device=dev)))
+    lflops = sum(2.0 / 3.0 * n ** 3 for n in ns)
+    lbytes = sum(8.0 * (n * n + 2 * n) for n in ns)
+    status = torch.empty(len(ns), dtype=torch.int32, device=dev)
+    for mode, name in ((0, "lu_20000_fp64"), (1, "lu_20000_fp32_ir")):
+        lp = bt.lu_plan(systems, mode, status)
+        ms = timed(lp)
+        ok = bool((status == 0).all())
+        out[name] = {"ms": round(ms, 4), "systems_per_s": round(len(ns) / ms * 1e3, 0),
+                     "gbps_matrices": round(lbytes / ms / 1e6, 1), "gflops": round(lflops / ms / 1e6, 1),
+                     "all_nonsingular": ok}
+        lp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default=None,
-                    choices=sorted(WORKLOADS) + ["ozaki_8192", "lu_ir_16384", "copy_1gib"])
+                    choices=sorted(WORKLOADS) + ["ozaki_8192", "lu_ir_16384", "batched", "copy_1gib"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rows", action="store_true", help="headline GEMM only")
     args = ap.parse_args()
@@ -290,6 +355,9 @@ def main():
         return copy_calibration(args)
     if args.workload == "ozaki_8192":
         print(json.dumps({"workload": "ozaki_8192", **ozaki_row(lib)}), flush=True)
+        return
+    if args.workload == "batched":
+        print(json.dumps({"workload": "batched", **batched_row(lib)}), flush=True)
         return
     if args.workload == "lu_ir_16384":
         print(json.dumps({"workload": "lu_ir_16384", **lu_ir_row(lib)}), flush=True)
@@ -336,6 +404,7 @@ def main():
             if w.startswith("syrk"):
                 rows[w]["hbm_gbps_algorithmic"] = round((q["M"] * q["K"] + q["M"] * q["N"]) * 8 /
                                                         (q["ms_per_step"] * 1e-3) / 1e9, 1)
+        rows["batched"] = batched_row(lib)
         rows["ozaki_8192"] = ozaki_row(lib)
         rows["lu_ir_16384"] = lu_ir_row(lib)
         out["rows"] = rows

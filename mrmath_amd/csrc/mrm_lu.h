@@ -8,7 +8,9 @@
 
 namespace mrm {
 
-constexpr int LU_NB = 256;             // outer block: K of the trailing-update GEMM
+constexpr int LU_NB = 256;             // outer block: K of the trailing-update GEMM (FP64)
+constexpr int LU_NB_LOWP = 512;        // outer block of the low-precision trailing update
+inline int lu_outer_block(int mode) { return mode == MRM_SOLVE_FP64 ? LU_NB : LU_NB_LOWP; }
 constexpr int LU_W = 32;               // panel width of the cooperative panel kernel
 constexpr int LU_TRSM_R = 64;          // rows per diagonal block of the blocked triangular solves
 constexpr int LU_MAX_PANEL_WG = 128;   // panel workgroups (all resident; candidates fit in LDS)

@@ -10,8 +10,9 @@
 // columns feed the FP64 MFMA GEMM with K = 256 for the trailing update; each
 // outer block is factored in panels of LU_W = 32 columns by one cooperative
 // kernel that keeps the panel's rows in registers, spread over as many
-// workgroups as the panel height needs, and takes one grid-wide barrier per
-// column to agree on the pivot.  The pivot rule is the reference's: the first
+// workgroups as the panel height needs; per column every workgroup publishes
+// its candidate in flagged words and polls everybody else's (one memory round
+// trip, no grid barrier) to agree on the pivot.  The pivot rule is the reference's: the first
 // row (in the current row order) of maximal |a| in the column, singular when
 // |pivot| <= 10/MaxDouble; the column is scaled by the reciprocal of the pivot.
 // The pivot sequence is recorded both as indx (row i was exchanged with row
@@ -65,85 +66,78 @@ __device__ __forceinline__ void wave_argmax(T& v, int& r) {
     }
 }
 
-// Grid-wide barrier for a cooperative launch: monotonic arrival counter.
-// One release fence before arriving and one acquire fence after the wait
-// (polling itself uses relaxed loads, so it does not invalidate caches on
-// every poll).  Every workgroup leaves after `deadline` (wall clock, 100 MHz)
-// at the latest and then reports MRM_LU_TIMEOUT, so no wave spins for ever.
-__device__ __forceinline__ bool grid_barrier(unsigned* counter, unsigned target, uint64_t deadline,
-                                             int* status) {
-    __syncthreads();
-    __shared__ int s_ok;
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned polls = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if ((++polls & 63u) == 0 && wall_clock64() > deadline) {
-                atomicExch(status, MRM_LU_TIMEOUT);
-                ok = 0;
-                break;
-            }
+// Cross-workgroup exchange of the panel: every 8-byte word written carries 32
+// bits of data and a 32-bit flag naming the column it belongs to, so a reader
+// polls the words it needs until their flags match and needs no barrier, no
+// counter and no cache maintenance: one memory round trip per column.  Aligned
+// 8-byte stores and loads are single-copy atomic; the words are relaxed
+// agent-scope atomics (coherent across the XCDs).  Two parities alternate by
+// column: a workgroup can only publish column j+2 after every workgroup has
+// published j+1, which each did after reading all of column j.
+template <typename T>
+struct Unit {
+    static constexpr int per = int(sizeof(T) / 4);  // 32-bit units per value
+    static __device__ __forceinline__ uint32_t part(T v, int h) {
+        if constexpr (sizeof(T) == 8) {
+            const uint64_t b = uint64_t(__double_as_longlong(v));
+            return uint32_t(h ? (b >> 32) : b);
+        } else {
+            return __float_as_uint(v);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        s_ok = ok;
     }
-    __syncthreads();
-    return s_ok != 0;
-}
+    static __device__ __forceinline__ T value(const uint32_t* u) {
+        if constexpr (sizeof(T) == 8) return __longlong_as_double((long long)((uint64_t(u[1]) << 32) | u[0]));
+        else return __uint_as_float(u[0]);
+    }
+};
+
+// Record of one workgroup for one column: [|candidate| (per units)][row]
+// [candidate row (W values)]; after the G records, row j (W values) from the
+// workgroup that owns it.
+template <typename T, int W>
+struct Rec {
+    static constexpr int UPV = Unit<T>::per;
+    static constexpr int RS = UPV + 1 + W * UPV;   // units per record
+    static constexpr int DU = W * UPV;             // units of row j
+    static __host__ __device__ constexpr size_t parity_units(int G) { return size_t(G) * RS + DU; }
+};
 
 template <typename T, int W>
 struct PanelShared {
-    T prow[W];                  // pivot row of the current column
-    T drow[W];                  // row j before the exchange
     T mine[W];                  // this workgroup's candidate row
+    T drow[W];                  // row j before the exchange (its owner's copy)
+    T pv[W];                    // the chosen pivot row, decoded
+    T rj[W];                    // row j, decoded
     T wv[PANEL_THREADS / 64];   // per-wave candidates
     int wr[PANEL_THREADS / 64];
-    T cv[mrm::LU_MAX_PANEL_WG];      // every workgroup's candidate |value|,
-    int cr[mrm::LU_MAX_PANEL_WG];    //   its row,
-    T cd[mrm::LU_MAX_PANEL_WG * W];  //   and the whole candidate row
+    int bad;                    // a poll ran past the deadline
+    uint32_t raw[mrm::LU_MAX_PANEL_WG * Rec<T, W>::RS + Rec<T, W>::DU];  // every record of the column
 };
 
-// Exchange area of one column parity: [G values][G rows][G*W row data][W diag row].
-template <typename T, int W>
-struct Exchange {
-    T* val;
-    int* row;
-    T* data;
-    T* diag;
-    __device__ Exchange(T* xch, int G, int par) {
-        const size_t per = size_t(G) * (W + 2) + W;
-        T* base = xch + par * per;
-        val = base;
-        row = reinterpret_cast<int*>(base + G);
-        data = base + 2 * G;
-        diag = base + 2 * G + size_t(G) * W;
-    }
-};
-
-template <typename T>
-__device__ __forceinline__ void st_agent(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_unit(uint64_t* p, uint32_t flag, uint32_t v) {
+    __hip_atomic_store(p, (uint64_t(flag) << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <typename T>
-__device__ __forceinline__ T ld_agent(T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint64_t ld_unit(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One column j of the panel (j is a compile-time constant after unrolling, so
-// v stays in registers).  Critical path: local argmax, parallel publication of
-// the candidate row (and of row j by its owner), one grid barrier, one parallel
-// read of all candidates into LDS, redundant selection by every thread.
-// Returns false when the grid stops (singular column or barrier timeout).
+// v stays in registers).  Critical path: local argmax, publication of the
+// workgroup's record (and of row j by its owner), one polled read of all
+// records into LDS, redundant selection by every thread.  Returns false when
+// the grid stops (singular column or a poll past the deadline).
 template <typename T, int W, int RPT>
 __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const int (&lr)[RPT], int m, int g, int G,
                                              int t, int lane, int wave, int piv_base, int* __restrict__ indx,
-                                             int* __restrict__ perm, T* __restrict__ xch,
-                                             unsigned* __restrict__ counter, int* __restrict__ status,
-                                             uint64_t deadline, PanelShared<T, W>& sh) {
+                                             uint64_t* __restrict__ xch, uint32_t flag_base,
+                                             int* __restrict__ status, uint64_t deadline, PanelShared<T, W>& sh) {
     constexpr int NW = PANEL_THREADS / 64;
-    Exchange<T, W> x(xch, G, j & 1);
+    using R = Rec<T, W>;
+    constexpr int UPV = R::UPV;
+    static_assert(R::RS <= 128 && R::DU <= PANEL_THREADS - 128, "record does not fit the publishing threads");
+    const uint32_t flag = flag_base + uint32_t(j);
+    uint64_t* recs = xch + size_t(j & 1) * R::parity_units(G);
+    uint64_t* diag = recs + size_t(G) * R::RS;
     // (a) best candidate of this workgroup among rows j..m-1
     T best = T(-1);
     int brow = INT_MAX;
@@ -185,65 +179,99 @@ __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const 
         }
     }
     __syncthreads();
-    // ... and the workgroup publishes them with one store per thread
-    if (t < W) {
-        st_agent(&x.data[size_t(g) * W + t], sh.mine[t]);
-        if (own_j) st_agent(&x.diag[t], sh.drow[t]);
-    } else if (t == W) {
-        st_agent(&x.val[g], best);
-    } else if (t == W + 1) {
-        st_agent(&x.row[g], brow);
+    // ... and publish them, one flagged word per thread
+    if (t < R::RS) {
+        uint32_t u;
+        if (t < UPV) u = Unit<T>::part(best, t);
+        else if (t == UPV) u = uint32_t(brow);
+        else u = Unit<T>::part(sh.mine[(t - UPV - 1) / UPV], (t - UPV - 1) % UPV);
+        st_unit(recs + size_t(g) * R::RS + t, flag, u);
+    } else if (own_j && t >= 128 && t < 128 + R::DU) {
+        const int e = t - 128;
+        st_unit(diag + e, flag, Unit<T>::part(sh.drow[e / UPV], e % UPV));
     }
-    // (c) all workgroups agree
-    if (!grid_barrier(counter, unsigned(G) * unsigned(j + 1), deadline, status)) return false;
-    // (d) one parallel read of every candidate (and of row j) into LDS
-    for (int q = t; q < G; q += PANEL_THREADS) {
-        sh.cv[q] = ld_agent(&x.val[q]);
-        sh.cr[q] = ld_agent(&x.row[q]);
-    }
-    for (int e = t; e < G * W; e += PANEL_THREADS) sh.cd[e] = ld_agent(&x.data[e]);
-    if (!own_j && t < W) sh.drow[t] = ld_agent(&x.diag[t]);
-    __syncthreads();
-    // every thread selects the pivot itself: max |value|, earliest row on ties
-    T gb = sh.cv[0];
-    int gr = sh.cr[0], gw = 0;
-    for (int q = 1; q < G; ++q)
-        if (better(sh.cv[q], sh.cr[q], gb, gr)) {
-            gb = sh.cv[q];
-            gr = sh.cr[q];
-            gw = q;
+    // (c) poll every record (and row j) into LDS
+    const int nrec = G * R::RS;
+    const int total = nrec + (own_j ? 0 : R::DU);
+    bool late = false;
+    for (int e = t; e < total; e += PANEL_THREADS) {
+        const uint64_t* p = e < nrec ? recs + e : diag + (e - nrec);
+        uint64_t u = ld_unit(p);
+        unsigned polls = 0;
+        while (uint32_t(u >> 32) != flag) {
+            if ((++polls & 63u) == 0 && wall_clock64() > deadline) {
+                late = true;
+                break;
+            }
+            u = ld_unit(p);
         }
+        sh.raw[e] = uint32_t(u);
+    }
+    if (late) {
+        sh.bad = 1;
+        atomicExch(status, MRM_LU_TIMEOUT);
+    }
+    __syncthreads();
+    if (sh.bad) return false;
+    // every wave selects the pivot itself (lane q looks at record q): max
+    // |value|, earliest row on ties
+    T gb = T(-1);
+    int gr = INT_MAX;
+    for (int q = lane; q < G; q += 64) {
+        const T cv = Unit<T>::value(&sh.raw[q * R::RS]);
+        const int cr = int(sh.raw[q * R::RS + UPV]);
+        if (better(cv, cr, gb, gr)) {
+            gb = cv;
+            gr = cr;
+        }
+    }
+    wave_argmax(gb, gr);
     if (!(gb > Lim<T>::tiny())) {
         if (g == 0 && t == 0) atomicExch(status, MRM_SINGULAR);
         return false;
     }
     const int prow = gr;
-    const T* pv = &sh.cd[gw * W];
+    // the record holding the pivot: the workgroup that owns row prow
+    const int gw = prow / (RPT * PANEL_THREADS);
     if (g == 0 && t == 0) indx[piv_base + j] = piv_base + prow;
-    // (e) exchange rows j and prow
+    // decode the pivot row and row j once into LDS
+    if (t < W) {
+        sh.pv[t] = Unit<T>::value(&sh.raw[gw * R::RS + UPV + 1 + t * UPV]);
+    } else if (t < 2 * W) {
+        const int c = t - W;
+        sh.rj[c] = own_j ? sh.drow[c] : Unit<T>::value(&sh.raw[nrec + c * UPV]);
+    }
+    __syncthreads();
+    // (d) exchange rows j and prow
     if (prow != j) {
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             if (lr[r] == prow) {
 #pragma unroll
-                for (int c = 0; c < W; ++c) v[r][c] = sh.drow[c];
+                for (int c = 0; c < W; ++c) v[r][c] = sh.rj[c];
             } else if (lr[r] == j) {
 #pragma unroll
-                for (int c = 0; c < W; ++c) v[r][c] = pv[c];
+                for (int c = 0; c < W; ++c) v[r][c] = sh.pv[c];
             }
         }
     }
-    // (f) scale by the reciprocal pivot (src/LinAlgLU.pas:264) and update the panel row
-    const T inv = T(1) / pv[j];
+    // (e) scale by the reciprocal pivot (src/LinAlgLU.pas:264) and update the panel row
+    const T inv = T(1) / sh.pv[j];
+    T pvc[W];
+#pragma unroll
+    for (int c = j + 1; c < W; ++c) pvc[c] = sh.pv[c];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         if (lr[r] > j && lr[r] < m) {
             const T l = v[r][j] * inv;
             v[r][j] = l;
 #pragma unroll
-            for (int c = j + 1; c < W; ++c) v[r][c] = fma(-l, pv[c], v[r][c]);
+            for (int c = j + 1; c < W; ++c) v[r][c] = fma(-l, pvc[c], v[r][c]);
         }
     }
+    // the next column writes sh.mine / sh.drow only after its first
+    // __syncthreads and sh.pv / sh.rj after its second, when every thread is
+    // done with this column's values
     return true;
 }
 
@@ -251,20 +279,20 @@ __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const 
 template <typename T, int W, int RPT, int J>
 struct PanelColumns {
     static __device__ __forceinline__ void run(int w, bool& alive, T (&v)[RPT][W], const int (&lr)[RPT], int m, int g,
-                                               int G, int t, int lane, int wave, int piv_base, int* indx, int* perm,
-                                               T* xch, unsigned* counter, int* status, uint64_t deadline,
+                                               int G, int t, int lane, int wave, int piv_base, int* indx,
+                                               uint64_t* xch, uint32_t flag_base, int* status, uint64_t deadline,
                                                PanelShared<T, W>& sh) {
         if (J < w && alive)
-            alive = panel_column<T, W, RPT>(J, v, lr, m, g, G, t, lane, wave, piv_base, indx, perm, xch, counter,
+            alive = panel_column<T, W, RPT>(J, v, lr, m, g, G, t, lane, wave, piv_base, indx, xch, flag_base, status,
+                                            deadline, sh);
+        PanelColumns<T, W, RPT, J + 1>::run(w, alive, v, lr, m, g, G, t, lane, wave, piv_base, indx, xch, flag_base,
                                             status, deadline, sh);
-        PanelColumns<T, W, RPT, J + 1>::run(w, alive, v, lr, m, g, G, t, lane, wave, piv_base, indx, perm, xch,
-                                            counter, status, deadline, sh);
     }
 };
 template <typename T, int W, int RPT>
 struct PanelColumns<T, W, RPT, W> {
     static __device__ __forceinline__ void run(int, bool&, T (&)[RPT][W], const int (&)[RPT], int, int, int, int, int,
-                                               int, int, int*, int*, T*, unsigned*, int*, uint64_t,
+                                               int, int, int*, uint64_t*, uint32_t, int*, uint64_t,
                                                PanelShared<T, W>&) {}
 };
 
@@ -274,8 +302,7 @@ struct PanelColumns<T, W, RPT, W> {
 template <typename T, int W, int RPT>
 __global__ void __launch_bounds__(PANEL_THREADS)
 lu_panel_kernel(T* __restrict__ P, int64_t lda, int m, int w, int piv_base, int* __restrict__ indx,
-                int* __restrict__ perm, T* __restrict__ xch, unsigned* __restrict__ counter,
-                int* __restrict__ status, uint64_t timeout_ticks) {
+                uint64_t* __restrict__ xch, uint32_t flag_base, int* __restrict__ status, uint64_t timeout_ticks) {
     const int G = gridDim.x;
     const int g = blockIdx.x;
     const int t = threadIdx.x;
@@ -283,6 +310,7 @@ lu_panel_kernel(T* __restrict__ P, int64_t lda, int m, int w, int piv_base, int*
     __shared__ PanelShared<T, W> sh;
     // an earlier panel found the matrix singular: nothing left to do (uniform)
     if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    if (t == 0) sh.bad = 0;
 
     T v[RPT][W];
     int lr[RPT];
@@ -297,8 +325,8 @@ lu_panel_kernel(T* __restrict__ P, int64_t lda, int m, int w, int piv_base, int*
     const uint64_t deadline = wall_clock64() + timeout_ticks;
     bool alive = true;
 
-    PanelColumns<T, W, RPT, 0>::run(w, alive, v, lr, m, g, G, t, lane, wave, piv_base, indx, perm, xch, counter,
-                                    status, deadline, sh);
+    PanelColumns<T, W, RPT, 0>::run(w, alive, v, lr, m, g, G, t, lane, wave, piv_base, indx, xch, flag_base, status,
+                                    deadline, sh);
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         if (lr[r] < m) {
@@ -421,7 +449,6 @@ trsm_upper_block_kernel(const T* __restrict__ U, int64_t ldu, T* __restrict__ X,
         if (i < rows) X[int64_t(i) * ldx + col] = x[i];
 }
 
-// Y[i][c] = B[perm[i]][c]
 // X += D (rows x cols, element strides)
 template <typename T>
 __global__ void add_inplace_kernel(T* __restrict__ X, int64_t ldx, const T* __restrict__ D, int64_t ldd, int rows,
@@ -449,9 +476,8 @@ namespace mrm {
 template <typename T>
 int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm, LuWork& ws, hipStream_t s) {
     constexpr int W = LU_W;
-    // A grid barrier costs ~2 us with 16 workgroups and ~4.6 us with 64
-    // (measured with the candidate exchange), so rows are packed up to 4 per
-    // thread before more workgroups are added.
+    // Every column reads every workgroup's record, so rows are packed up to 4
+    // per thread before more workgroups are added.
     int rpt = 1;
     int G = grid1(m, PANEL_THREADS);
     while (G > LU_PANEL_WG_TARGET && rpt < 4) {
@@ -462,10 +488,12 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
         mrm_set_error(MRM_BAD_DIMS, "lu: matrix too tall for the panel kernel (%d rows)", m);
         return MRM_BAD_DIMS;
     }
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ws.counter, 1);
-    T* xch = reinterpret_cast<T*>(ws.xch);
+    uint64_t* xch = static_cast<uint64_t*>(ws.xch);
+    // flags are unique within a factorisation (the exchange area is zeroed at
+    // its start): column j of this panel is flagged piv_base + j + 1
+    uint32_t flag_base = uint32_t(piv_base) + 1u;
     uint64_t timeout = 2ull * 100000000ull;  // 2 s of wall clock (100 MHz)
-    void* args[] = {&P, &lda, &m, &w, &piv_base, &indx, &perm, &xch, &ws.counter, &ws.status, &timeout};
+    void* args[] = {&P, &lda, &m, &w, &piv_base, &indx, &xch, &flag_base, &ws.status, &timeout};
     hipError_t e;
     const void* fn = rpt == 1   ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 1>)
                      : rpt == 2 ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 2>)
@@ -527,8 +555,14 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
     // indx starts at 0 as in the reference (src/LinAlgLU.pas:289); entries a
     // singular stop leaves unset stay in range for the row interchanges
     zero_i32(indx, n, s);
-    for (int J = 0; J < n; J += LU_NB) {
-        const int jb = n - J < LU_NB ? n - J : LU_NB;
+    // no flag of an earlier factorisation may survive in the exchange area
+    const int xw = int(lu_xch_bytes() / 4);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(grid1(xw, 256)), dim3(256), 0, s, static_cast<unsigned*>(ws.xch), xw);
+    // the low-precision path takes twice the outer block: half as many
+    // read-modify-write passes of the FP64 trailing matrix
+    const int nb = lu_outer_block(mode);
+    for (int J = 0; J < n; J += nb) {
+        const int jb = n - J < nb ? n - J : nb;
         for (int j = J; j < J + jb; j += LU_W) {
             const int wb = J + jb - j < LU_W ? J + jb - j : LU_W;
             int rc = lu_panel<T>(A + int64_t(j) * lda + j, lda, n - j, wb, j, indx, perm, ws, s);
@@ -574,8 +608,9 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
 template int lu_factor<double>(double*, int64_t, int, int*, int*, LuWork&, hipStream_t, int);
 
 size_t lu_lowp_bytes(int n, int mode) {
-    if (mode == MRM_SOLVE_FP64 || n <= LU_NB) return 0;
-    return mrm_lowp_workspace(n - LU_NB, n - LU_NB, LU_NB, mode);
+    const int nb = lu_outer_block(mode);
+    if (mode == MRM_SOLVE_FP64 || n <= nb) return 0;
+    return mrm_lowp_workspace(n - nb, n - nb, nb, mode);
 }
 
 int lu_lowp_reserve(LuWork& ws, int n, int mode) {
@@ -605,8 +640,8 @@ void zero_i32(int* p, int n, hipStream_t s) {
 }
 
 size_t lu_xch_bytes() {
-    // two parities of [G values][G rows][G*W data][W diag], doubles, G <= LU_MAX_PANEL_WG
-    return 2 * (size_t(LU_MAX_PANEL_WG) * (LU_W + 2) + LU_W) * sizeof(double);
+    // two parities of G flagged records plus row j, 8-byte words, G <= LU_MAX_PANEL_WG
+    return 2 * Rec<double, LU_W>::parity_units(LU_MAX_PANEL_WG) * sizeof(uint64_t);
 }
 
 }  // namespace mrm

@@ -10,8 +10,9 @@
 //     problems form one list, and a persistent grid of workgroups takes tiles
 //     from an atomic counter (largest K first).  T in {32, 64, 128} is picked
 //     by timing the actual batch (the autotuner) unless the caller fixes it.
-//   LU solve: systems are grouped by size class (n <= 32, 64, 128); per class
-//     one persistent launch where each wave takes whole systems from a counter,
+//   LU solve: systems are grouped by size class (n <= 32, 64, 96, 128); per class
+//     one persistent launch where each workgroup (one wave up to 64 unknowns,
+//     four above) takes whole systems from a counter,
 //     holds A in LDS, factors it with partial pivoting (the reference's rule:
 //     first row of maximal |a|, singular at |pivot| <= 10/MaxDouble) and
 //     substitutes.  FP64, or FP32 factors with FP64 residuals and refinement
@@ -161,17 +162,25 @@ __device__ __forceinline__ double tiny_pivot<double>() { return 10.0 / DBL_MAX; 
 template <>
 __device__ __forceinline__ float tiny_pivot<float>() { return 10.0f / FLT_MAX; }
 
-// LU with partial pivoting of the n x n matrix L (row stride ld) in LDS by one
-// wave; rows are exchanged physically, piv[j] records the row exchanged with j.
-// Returns false when singular.
-template <typename T>
-__device__ bool wave_lu(T* L, int ld, int n, int* piv, int lane) {
-    bool ok = true;
+// Block-wide sync for NWV waves (NWV == 1: the wave's own LDS ordering).
+template <int NWV>
+__device__ __forceinline__ void blk_sync() {
+    if constexpr (NWV == 1) wave_sync();
+    else __syncthreads();
+}
+
+// LU with partial pivoting of the n x n matrix L (row stride ld) in LDS by NWV
+// waves; rows are exchanged physically, piv[j] records the row exchanged with
+// j.  Returns false when singular.  wv / wr: NWV slots of LDS scratch.
+template <typename T, int NWV>
+__device__ bool block_lu(T* L, int ld, int n, int* piv, int tid, T* wv, int* wr) {
+    constexpr int NT = 64 * NWV;
+    const int lane = tid & 63, wave = tid >> 6;
     for (int j = 0; j < n; ++j) {
         // first row of maximal |a| in column j
         T best = T(-1);
         int brow = n;
-        for (int i = j + lane; i < n; i += 64) {
+        for (int i = j + tid; i < n; i += NT) {
             const T v = fabs(L[i * ld + j]);
             if (v > best) { best = v; brow = i; }
         }
@@ -181,27 +190,51 @@ __device__ bool wave_lu(T* L, int ld, int n, int* piv, int lane) {
             const int orow = __shfl_xor(brow, off, 64);
             if (ov > best || (ov == best && orow < brow)) { best = ov; brow = orow; }
         }
-        if (lane == 0) piv[j] = brow;
-        if (!(best > tiny_pivot<T>())) { ok = false; break; }
+        if constexpr (NWV > 1) {
+            if (lane == 0) {
+                wv[wave] = best;
+                wr[wave] = brow;
+            }
+            __syncthreads();
+            best = wv[0];
+            brow = wr[0];
+#pragma unroll
+            for (int q = 1; q < NWV; ++q)
+                if (wv[q] > best || (wv[q] == best && wr[q] < brow)) { best = wv[q]; brow = wr[q]; }
+        }
+        if (tid == 0) piv[j] = brow;
+        if (!(best > tiny_pivot<T>())) return false;  // uniform
         if (brow != j)
-            for (int k = lane; k < n; k += 64) {
+            for (int k = tid; k < n; k += NT) {
                 const T t = L[j * ld + k];
                 L[j * ld + k] = L[brow * ld + k];
                 L[brow * ld + k] = t;
             }
-        wave_sync();
+        blk_sync<NWV>();
         const T rinv = T(1) / L[j * ld + j];
-        for (int i = j + 1 + lane; i < n; i += 64) L[i * ld + j] *= rinv;
-        wave_sync();
-        const int w = n - j - 1;
-        for (int q = lane; q < w * w; q += 64) {
-            const int i = j + 1 + q / w, k = j + 1 + q % w;
-            L[i * ld + k] -= L[i * ld + j] * L[j * ld + k];
+        for (int i = j + 1 + tid; i < n; i += NT) L[i * ld + j] *= rinv;
+        blk_sync<NWV>();
+        // thread owns column k = j+1+lane (+64) and the rows i = j+1+wave
+        // (+NWV...); L[i][j] is the same for all lanes of a wave (broadcast).
+        // Rows go in groups of 8 whose loads are issued before the stores.
+        for (int k = j + 1 + lane; k < n; k += 64) {
+            const T ujk = L[j * ld + k];
+            for (int i0 = j + 1 + wave; i0 < n; i0 += 8 * NWV) {
+                T lij[8], lik[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int i = i0 + u * NWV < n ? i0 + u * NWV : n - 1;
+                    lij[u] = L[i * ld + j];
+                    lik[u] = L[i * ld + k];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (i0 + u * NWV < n) L[(i0 + u * NWV) * ld + k] = lik[u] - lij[u] * ujk;
+            }
         }
-        wave_sync();
+        blk_sync<NWV>();
     }
-    wave_sync();
-    return ok;
+    return true;
 }
 
 // y := (LU)^-1 P y in place (y in LDS, type Y), with the factors of wave_lu.
@@ -228,58 +261,86 @@ __device__ void wave_lu_subst(const T* L, int ld, int n, const int* piv, Y* y, i
 }
 
 // MODE 0: FP64 factors.  MODE 1: FP32 factors, `refine` FP64 refinement steps.
-// Dynamic LDS per wave: nmax*(nmax+1) factors of type T, 2*nmax doubles, nmax ints.
-template <int MODE>
-__global__ void __launch_bounds__(64) bl_solve_kernel(const LuP* __restrict__ sys, int count, int nmax,
-                                                      unsigned* __restrict__ counter, int* __restrict__ status,
-                                                      int refine) {
+// NWV waves per system (the factorisation is shared by all, the
+// substitutions run on wave 0).  Dynamic LDS per workgroup: nmax*(nmax+1)
+// factors of type T, 2*nmax doubles, nmax ints.
+template <int MODE, int NWV>
+__global__ void __launch_bounds__(64 * NWV) bl_solve_kernel(const LuP* __restrict__ sys, int count, int nmax,
+                                                            unsigned* __restrict__ counter, int* __restrict__ status,
+                                                            int refine) {
     using T = typename std::conditional<MODE == 0, double, float>::type;
+    constexpr int NT = 64 * NWV;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    const int lane = threadIdx.x;
+    __shared__ T wv[NWV];
+    __shared__ int wr[NWV];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     T* L = reinterpret_cast<T*>(lds);
     double* xv = reinterpret_cast<double*>(lds + ((size_t(nmax) * (nmax + 1) * sizeof(T) + 15) & ~size_t(15)));
     double* rv = xv + nmax;
     int* piv = reinterpret_cast<int*>(rv + nmax);
     for (;;) {
-        int item = 0;
-        if (lane == 0) item = int(atomicAdd(counter, 1u));
-        item = __shfl(item, 0, 64);
+        if (tid == 0) s_item = int(atomicAdd(counter, 1u));
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
         if (item >= count) return;
         const LuP s = sys[item];
         const int n = int(s.n), ld = n + 1;
-        for (int q = lane; q < n * n; q += 64) L[(q / n) * ld + q % n] = T(s.A[(q / n) * s.lda + q % n]);
-        for (int i = lane; i < n; i += 64) xv[i] = s.b[i];
-        wave_sync();
-        const bool ok = wave_lu<T>(L, ld, n, piv, lane);
+        // A into LDS, 16 rows (per wave) x 2 columns per lane in flight per
+        // group: the loads of a group are issued before any of its LDS stores
+        for (int r0 = wave * 16; r0 < n; r0 += 16 * NWV) {
+            double a[16][2];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = r0 + u, c = lane + 64 * h;
+                    a[u][h] = (r < n && c < n) ? s.A[int64_t(r) * s.lda + c] : 0.0;
+                }
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int r = r0 + u, c = lane + 64 * h;
+                    if (r < n && c < n) L[r * ld + c] = T(a[u][h]);
+                }
+        }
+        for (int i = tid; i < n; i += NT) xv[i] = s.b[i];
+        __syncthreads();
+        const bool ok = block_lu<T, NWV>(L, ld, n, piv, tid, wv, wr);
         if (ok) {
             if (MODE == 0) {
-                wave_lu_subst<T, double>(L, ld, n, piv, xv, lane);
+                if (wave == 0) wave_lu_subst<T, double>(L, ld, n, piv, xv, lane);
+                __syncthreads();
             } else {
                 // FP32 solve of the FP64 right-hand side, then FP64 refinement
                 float* yv = reinterpret_cast<float*>(rv);
-                for (int i = lane; i < n; i += 64) yv[i] = float(xv[i]);
-                wave_sync();
-                wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
-                for (int i = lane; i < n; i += 64) xv[i] = double(yv[i]);
-                wave_sync();
+                for (int i = tid; i < n; i += NT) yv[i] = float(xv[i]);
+                __syncthreads();
+                if (wave == 0) wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
+                __syncthreads();
+                for (int i = tid; i < n; i += NT) xv[i] = double(yv[i]);
+                __syncthreads();
                 for (int it = 0; it < refine; ++it) {
-                    // r = b - A x with A and b from memory in FP64 (row i per lane)
-                    for (int i = lane; i < n; i += 64) {
+                    // r = b - A x with A and b from memory in FP64 (row i per thread)
+                    for (int i = tid; i < n; i += NT) {
                         double r = s.b[i];
                         const double* arow = s.A + int64_t(i) * s.lda;
                         for (int k = 0; k < n; ++k) r -= arow[k] * xv[k];
-                        yv[i] = float(r);  // lanes own disjoint rows
+                        yv[i] = float(r);  // threads own disjoint rows
                     }
-                    wave_sync();
-                    wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
-                    for (int i = lane; i < n; i += 64) xv[i] += double(yv[i]);
-                    wave_sync();
+                    __syncthreads();
+                    if (wave == 0) wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
+                    __syncthreads();
+                    for (int i = tid; i < n; i += NT) xv[i] += double(yv[i]);
+                    __syncthreads();
                 }
             }
-            for (int i = lane; i < n; i += 64) s.x[i] = xv[i];
+            for (int i = tid; i < n; i += NT) s.x[i] = xv[i];
         }
-        if (lane == 0 && status) status[s.idx] = ok ? MRM_OK : MRM_SINGULAR;
-        wave_sync();
+        if (tid == 0 && status) status[s.idx] = ok ? MRM_OK : MRM_SINGULAR;
+        __syncthreads();
     }
 }
 
@@ -307,9 +368,10 @@ struct Plan {
     GemmP* probs = nullptr;
     Tile* tiles = nullptr;
     int ntiles = 0;
-    LuP* sys[3] = {nullptr, nullptr, nullptr};
-    int counts[3] = {0, 0, 0};
-    int nmax[3] = {32, 64, 128};
+    static constexpr int NCLS = 4;
+    LuP* sys[NCLS] = {nullptr, nullptr, nullptr, nullptr};
+    int counts[NCLS] = {0, 0, 0, 0};
+    int nmax[NCLS] = {32, 64, 96, 128};
     int* status = nullptr;
     int refine = 2;
 };
@@ -359,17 +421,29 @@ size_t lu_lds(int nmax, int mode) {
     return ((fac + 15) & ~size_t(15)) + size_t(nmax) * 16 + size_t(nmax) * 4;
 }
 
+// Systems up to 64 unknowns get one wave each; larger ones four waves, which
+// share the factorisation (their matrix alone fills most of a CU's LDS).
+int lu_block(int cls) { return cls < 2 ? 64 : 256; }
+
+const void* lu_fn(int mode, int block) {
+    if (mode == 0)
+        return block == 64 ? reinterpret_cast<const void*>(&bl_solve_kernel<0, 1>)
+                           : reinterpret_cast<const void*>(&bl_solve_kernel<0, 4>);
+    return block == 64 ? reinterpret_cast<const void*>(&bl_solve_kernel<1, 1>)
+                       : reinterpret_cast<const void*>(&bl_solve_kernel<1, 4>);
+}
+
 int launch_lu(Plan& pl, hipStream_t s) {
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, pl.counters, 3);
-    for (int c = 0; c < 3; ++c) {
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, pl.counters, Plan::NCLS);
+    for (int c = 0; c < Plan::NCLS; ++c) {
         if (pl.counts[c] == 0) continue;
-        const void* fn = pl.mode == 0 ? reinterpret_cast<const void*>(&bl_solve_kernel<0>)
-                                      : reinterpret_cast<const void*>(&bl_solve_kernel<1>);
+        const int block = lu_block(c);
+        const void* fn = lu_fn(pl.mode, block);
         const size_t lds = lu_lds(pl.nmax[c], pl.mode);
-        const int grid = resident_grid(fn, 64, lds);
+        const int grid = resident_grid(fn, block, lds);
         unsigned* ctr = pl.counters + c;
         void* args[] = {&pl.sys[c], &pl.counts[c], &pl.nmax[c], &ctr, &pl.status, &pl.refine};
-        MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(64), args, lds, s));
+        MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, s));
     }
     return MRM_OK;
 }
@@ -502,7 +576,7 @@ int32_t mrm_batched_lu_plan(const mrm_lu_desc* systems, int64_t count, int32_t m
         return MRM_BAD_ARG;
     }
     *plan = nullptr;
-    std::vector<LuP> cls[3];
+    std::vector<LuP> cls[Plan::NCLS];
     for (int64_t i = 0; i < count; ++i) {
         const mrm_lu_desc& d = systems[i];
         if (d.n < 0 || d.n > 128 || (d.lda & 7) || (d.n > 0 && (d.lda < d.n * 8 || !d.A || !d.b || !d.x))) {
@@ -511,7 +585,7 @@ int32_t mrm_batched_lu_plan(const mrm_lu_desc* systems, int64_t count, int32_t m
             return MRM_BAD_DIMS;
         }
         const LuP p{d.n, d.lda / 8, d.A, d.b, d.x, int(i)};
-        cls[d.n <= 32 ? 0 : d.n <= 64 ? 1 : 2].push_back(p);
+        cls[d.n <= 32 ? 0 : d.n <= 64 ? 1 : d.n <= 96 ? 2 : 3].push_back(p);
     }
     Plan* pl = new Plan;
     pl->kind = 2;
@@ -532,9 +606,9 @@ int32_t mrm_batched_lu_plan(const mrm_lu_desc* systems, int64_t count, int32_t m
     }
     char* base = static_cast<char*>(pl->dev);
     pl->counters = reinterpret_cast<unsigned*>(base);
-    pl->ncounters = 3;
+    pl->ncounters = Plan::NCLS;
     size_t off = 256;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < Plan::NCLS; ++c) {
         pl->sys[c] = reinterpret_cast<LuP*>(base + off);
         pl->counts[c] = int(cls[c].size());
         if (!cls[c].empty() &&
@@ -543,12 +617,11 @@ int32_t mrm_batched_lu_plan(const mrm_lu_desc* systems, int64_t count, int32_t m
             rc = MRM_HIP_ERROR;
         off += align256(cls[c].size() * sizeof(LuP) + 1);
     }
-    for (int m : {0, 1}) {
-        const void* fn = m == 0 ? reinterpret_cast<const void*>(&bl_solve_kernel<0>)
-                                : reinterpret_cast<const void*>(&bl_solve_kernel<1>);
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lu_lds(128, m))) != hipSuccess)
-            rc = MRM_HIP_ERROR;
-    }
+    for (int m : {0, 1})
+        for (int block : {64, 256})
+            if (hipFuncSetAttribute(lu_fn(m, block), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lu_lds(128, m))) != hipSuccess)
+                rc = MRM_HIP_ERROR;
     pl->tile = 64;
     if (!rc) rc = capture(*pl, s);
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MRM_HIP_ERROR;

@@ -158,6 +158,16 @@ MRM_API int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t wid
                                 int64_t LineWidthB, double* X, int64_t LineWidthX, int64_t width2,
                                 int64_t NumRefinments);
 
+/* MatrixInverseInPlace (reference src/LinAlgLU.pas:386-427): A (width x
+ * width) is replaced by its inverse, column j being (LU)^-1 P e_j.  Returns
+ * MRM_OK (leOk) or MRM_SINGULAR (leSingular; A is then left unchanged). */
+MRM_API int32_t mrm_inverse_inplace(double* A, int64_t LineWidthA, int64_t width);
+
+/* MatrixDeterminant (reference src/LinAlgLU.pas:429-478): *det = the product
+ * of U's diagonal, negated once per row interchange, 0 for a singular matrix.
+ * Returns MRM_OK or MRM_SINGULAR (both set *det) or an error. */
+MRM_API int32_t mrm_determinant(const double* A, int64_t LineWidthA, int64_t width, double* det);
+
 /* Device LU of the n x n matrix A (stride lda bytes): indx as above; perm
  * (optional, n entries) receives the row permutation, row i of P*A being row
  * perm[i] of A.  Synchronises `stream` to return the status.  No reference

@@ -47,6 +47,8 @@ SIGNATURES = {
     "mrm_batched_run": (_i32, [_pv, _pv]),
     "mrm_batched_plan_tile": (_i32, [_pv]),
     "mrm_batched_destroy": (None, [_pv]),
+    "mrm_inverse_inplace": (_i32, [_pd, _i64, _i64]),
+    "mrm_determinant": (_i32, [_pd, _i64, _i64, C.POINTER(C.c_double)]),
     "mrm_lu_decomp_inplace": (_i32, [_pd, _i64, _i64, _pv]),
     "mrm_lu_backsubst": (_i32, [_pd, _i64, _i64, _pv, _pd, _i64]),
     "mrm_lineq_solve": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64]),

@@ -133,6 +133,90 @@ int32_t mrm_lu_decomp_inplace(double* A, int64_t LineWidthA, int64_t width, int3
     return rc;
 }
 
+namespace {
+
+__global__ void identity_kernel(double* X, int64_t n) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx < n * n) X[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
+}
+
+}  // namespace
+
+// MatrixInverseInPlace (src/LinAlgLU.pas:386-427)
+int32_t mrm_inverse_inplace(double* A, int64_t LineWidthA, int64_t width) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(width, LineWidthA))) return rc;
+    if (width == 0) return MRM_OK;
+    if (!A) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    const int64_t n = width;
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    void *dLU, *dE, *dX, *dI, *dP;
+    if ((rc = mrm_staging(0, size_t(n * n) * 8, &dLU))) return rc;
+    if ((rc = mrm_staging(1, size_t(n * n) * 8, &dE))) return rc;
+    if ((rc = mrm_staging(2, size_t(n * n) * 8, &dX))) return rc;
+    if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
+    if ((rc = mrm_staging(6, size_t(n) * 4, &dP))) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(dLU, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    rc = factor_and_status(static_cast<double*>(dLU), n, int(n), static_cast<int*>(dI), static_cast<int*>(dP), s);
+    if (rc) return rc;  // leSingular leaves A unchanged, as the reference does
+    hipLaunchKernelGGL(identity_kernel, dim3(unsigned((n * n + 255) / 256)), dim3(256), 0, s,
+                       static_cast<double*>(dE), n);
+    if ((rc = mrm::lu_solve<double>(static_cast<const double*>(dLU), n, int(n), static_cast<const int*>(dP),
+                                    static_cast<const double*>(dE), n, static_cast<double*>(dX), n, int(n), s)))
+        return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(A, size_t(LineWidthA), dX, size_t(n) * 8, size_t(n) * 8, size_t(n),
+                                 hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    return MRM_OK;
+}
+
+// MatrixDeterminant (src/LinAlgLU.pas:429-478)
+int32_t mrm_determinant(const double* A, int64_t LineWidthA, int64_t width, double* det) {
+    int rc = mrm_ensure_init();
+    if (rc) return rc;
+    if ((rc = check_square(width, LineWidthA))) return rc;
+    if (!det || (width > 0 && !A)) {
+        mrm_set_error(MRM_BAD_ARG, "null pointer");
+        return MRM_BAD_ARG;
+    }
+    if (width == 0) {
+        *det = 1.0;
+        return MRM_OK;
+    }
+    const int64_t n = width;
+    std::lock_guard<std::mutex> lk(mrm_host_mutex());
+    hipStream_t s = mrm_stream();
+    void *dLU, *dI;
+    if ((rc = mrm_staging(0, size_t(n * n) * 8, &dLU))) return rc;
+    if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
+    MRM_HIP_TRY(hipMemcpy2DAsync(dLU, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
+                                 hipMemcpyHostToDevice, s));
+    rc = factor_and_status(static_cast<double*>(dLU), n, int(n), static_cast<int*>(dI), nullptr, s);
+    if (rc == MRM_SINGULAR) {
+        *det = 0.0;
+        return MRM_SINGULAR;
+    }
+    if (rc) return rc;
+    std::vector<double> diag(static_cast<size_t>(n));
+    std::vector<int32_t> indx(static_cast<size_t>(n));
+    // the diagonal is a strided column: one double every (n+1) elements
+    MRM_HIP_TRY(hipMemcpy2DAsync(diag.data(), 8, dLU, size_t(n + 1) * 8, 8, size_t(n), hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipMemcpyAsync(indx.data(), dI, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    double d = 1.0;
+    for (int64_t i = 0; i < n; ++i)
+        if (indx[size_t(i)] != i) d = -d;
+    for (int64_t i = 0; i < n; ++i) d = d * diag[size_t(i)];
+    *det = d;
+    return MRM_OK;
+}
+
 // MatrixLUBackSubst (src/LinAlgLU.pas:315-385): one right-hand side, stride LineWidthB.
 int32_t mrm_lu_backsubst(const double* LUDecomp, int64_t LineWidthLU, int64_t width, const int32_t* indx, double* B,
                          int64_t LineWidthB) {

@@ -72,6 +72,25 @@ def MatrixLinEQSolveEx(A: np.ndarray, B: np.ndarray, X: np.ndarray, NumRefinment
     return _status(_lib.load().mrm_lineq_solve_ex(pa, lwa, w, pb, lwb, px, lwx, w2, NumRefinments, mode))
 
 
+def MatrixInverseInPlace(A: np.ndarray) -> int:
+    """A := A^-1 (src/LinAlgLU.pas:386-427). Returns leOk or leSingular (A unchanged)."""
+    p, w, h, lw = _mat(A, "A")
+    if w != h:
+        raise ValueError("Dimension error: A must be square")
+    return _status(_lib.load().mrm_inverse_inplace(p, lw, w))
+
+
+def MatrixDeterminant(A: np.ndarray) -> float:
+    """det(A) from the LU (src/LinAlgLU.pas:429-478); 0 for a singular matrix."""
+    import ctypes as C
+    p, w, h, lw = _mat(A, "A")
+    if w != h:
+        raise ValueError("Dimension error: A must be square")
+    d = C.c_double()
+    _status(_lib.load().mrm_determinant(p, lw, w, C.byref(d)))
+    return d.value
+
+
 def SolveLinEQ(A: np.ndarray, B: np.ndarray, NumRefinments: int = 0) -> np.ndarray:
     """TDoubleMatrix.SolveLinEQ (src/DblMatrix.pas:1343-1362): returns X, raises on singular."""
     X = np.zeros((A.shape[0], B.shape[1]))

@@ -166,3 +166,31 @@ def test_device_lu_and_solve(n):
         A0 = An.copy()
         rc, io = orc.lu_decomp_inplace(A0)
         assert np.array_equal(indx.cpu().numpy(), io)
+
+
+def test_inverse_and_determinant_kats():
+    """Test/TestLineEquations.pas:635-642 and :702-713, bit for bit."""
+    A = np.array(KAT["det"]["A"], dtype=np.float64).reshape(2, 2)
+    assert la.MatrixDeterminant(A) == KAT["det"]["det"]
+    Ai = np.array(KAT["inverse"]["A"], dtype=np.float64).reshape(2, 2)
+    assert la.MatrixInverseInPlace(Ai) == la.leOk
+    assert np.array_equal(Ai.ravel(), KAT["inverse"]["inv"])
+    S = np.array(KAT["singular"]["A"], dtype=np.float64).reshape(3, 3)
+    S0 = S.copy()
+    assert la.MatrixInverseInPlace(S) == la.leSingular
+    assert np.array_equal(S, S0)
+    assert la.MatrixDeterminant(S0) == 0.0
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 128, 129, 300])
+def test_inverse_and_determinant_vs_oracle(n):
+    rng = np.random.default_rng(900 + n)
+    A = rng.standard_normal((n, n)) + 0.5 * n ** 0.5 * np.eye(n)
+    Ag, Ao = A.copy(), A.copy()
+    assert la.MatrixInverseInPlace(Ag) == la.leOk
+    assert orc.inverse_inplace(Ao) == 0
+    # the factors round differently (blocked vs recursive updates)
+    assert np.max(np.abs(Ag - Ao)) <= 1e-10 * np.max(np.abs(Ao))
+    assert np.max(np.abs(A @ Ag - np.eye(n))) <= 1e-10
+    dg, do = la.MatrixDeterminant(A), orc.determinant(A.copy())
+    assert dg == do or abs(dg - do) <= 1e-12 * abs(do)

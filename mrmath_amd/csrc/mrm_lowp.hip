@@ -10,10 +10,11 @@
 //
 // Operands are first packed into 64-byte k blocks, [k/64 bytes][row][64 bytes]
 // (16 FP32 or 32 BF16 values of one row), A by rows and B by columns, rows
-// padded to the 256-row tile and k to the block with zeros; one K step of a
-// 256-row panel is then 16 KiB of contiguous memory, moved by direct
-// global->LDS loads through a 4-stage ring.  Workgroup tile 256 x 256, 4 waves
-// of 128 x 128 (4 x 4 blocks of 32 x 32 on v_mfma_f32_32x32x2_f32 or
+// padded to 256 rows and k to the block with zeros; one K step of a 256-row
+// panel is then 16 KiB of contiguous memory, moved by direct global->LDS loads
+// through a 3-stage ring.  Workgroup tile 128 x 256 (two per CU, so one
+// workgroup's FP64 read-modify-write of C overlaps the other's products), 4
+// waves of 64 x 128 (2 x 4 blocks of 32 x 32 on v_mfma_f32_32x32x2_f32 or
 // v_mfma_f32_32x32x16_bf16).
 
 #include <cstdint>
@@ -32,7 +33,8 @@ typedef __attribute__((address_space(3))) void* lptr_t;
 constexpr int LP_BK = 64;       // bytes per row per K step
 constexpr int LP_THREADS = 256;
 constexpr int LP_TILE = 256;
-constexpr int LP_STAGES = 4;
+constexpr int LP_STAGES = 3;
+constexpr int LP_PM = 2, LP_PN = 4;  // 128 x 256 tiles, two workgroups per CU
 
 // FP32 -> BF16, round to nearest even (finite inputs)
 __device__ __forceinline__ uint16_t to_bf16(float f) {
@@ -90,10 +92,10 @@ __device__ __forceinline__ void lp_tile_coords(int bid, int tiles_m, int tiles_n
 // v_mfma_f32_32x32x2_f32; BF16: 8 values, one v_mfma_f32_32x32x16_bf16).
 // Result register v of lane l is row 8(v/4) + 4(l/32) + v%4, column l%32.
 template <int MODE>
-__global__ void __launch_bounds__(LP_THREADS, 1)
+__global__ void __launch_bounds__(LP_THREADS, 2)
 lp_update_kernel(const uint8_t* __restrict__ PA, const uint8_t* __restrict__ PB, int64_t kbpa, int64_t kbpb,
                  int64_t steps, double* __restrict__ C, int64_t ldc, int64_t M, int64_t N, int tiles_m, int tiles_n) {
-    constexpr int PM = 4, PN = 4, S = LP_STAGES;
+    constexpr int PM = LP_PM, PN = LP_PN, S = LP_STAGES;
     constexpr int BM = 64 * PM, BN = 64 * PN;
     constexpr int LA = BM / 64, LB = BN / 64;
     constexpr int STAGE = (BM + BN) * LP_BK;
@@ -256,7 +258,7 @@ int mrm_launch_lowp_update(int mode, int64_t M, int64_t N, int64_t K, const doub
         hipLaunchKernelGGL(lp_pack_kernel<MRM_LOWP_BF16>, ga, dim3(256), 0, stream, A, lda, int64_t(1), M, K, PA, Mp);
         hipLaunchKernelGGL(lp_pack_kernel<MRM_LOWP_BF16>, gb, dim3(256), 0, stream, B, int64_t(1), ldb, N, K, PB, Np);
     }
-    const int tiles_m = int(Mp / LP_TILE), tiles_n = int(Np / LP_TILE);
+    const int tiles_m = int(Mp / (64 * LP_PM)), tiles_n = int(Np / (64 * LP_PN));
     const dim3 grid(unsigned(tiles_m) * unsigned(tiles_n));
     if (mode == MRM_LOWP_FP32)
         hipLaunchKernelGGL(lp_update_kernel<MRM_LOWP_FP32>, grid, dim3(LP_THREADS), 0, stream, PA, PB, Mp * 64,

@@ -281,7 +281,7 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
 #pragma unroll
         for (int p = 0; p < 4; ++p) a_cur[p] = As[panel_index<AKC>(arow + 16 * p, kq)];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) b_cur[q] = Bs[panel_index<BKC>(bcol + 4 * q, kq)];   // round 0
+        for (int q = 0; q < 4; ++q) b_cur[q] = Bs[panel_index<BKC>(bcol + 4 * q, kq)];   // K step 0
 #pragma unroll
         for (int gidx = 0; gidx < 16; ++gidx) {
             const int c = gidx & 3;

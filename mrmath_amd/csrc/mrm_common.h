@@ -13,6 +13,13 @@ typedef float mrm_f4 __attribute__((ext_vector_type(4)));
 // Internal launchers (all device pointers, byte strides, enqueue on `stream`).
 int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                      const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream);
+// The same with the K dimension cut into `splits` chunks (partial products
+// reduced in chunk order): more workgroups for short, skinny products.  work
+// (optional, splits * M * N doubles) holds the partial products; without it
+// they come from a stream-ordered allocation.
+int mrm_launch_dgemm_splitk(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                            const double* B, int64_t ldb, double* C, int64_t ldc, int op, int splits,
+                            double* work, hipStream_t stream);
 
 void mrm_set_error(int code, const char* fmt, ...);
 

@@ -388,9 +388,14 @@ __global__ void zero_matrix_kernel(double* C, int64_t ldc, int64_t M, int64_t N)
 
 }  // namespace
 
+namespace {
+
 // C (M x N) = / += / -= op(A) * op(B), K the inner dimension.  Strides in bytes.
-int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
-                     const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream) {
+// want_splits > 0 asks for that many K chunks (at least BK each), 0 lets the
+// shape decide; work (optional) holds want_splits * M * N doubles of partial
+// products, otherwise they come from a stream-ordered allocation.
+int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
+                 int64_t ldb, double* C, int64_t ldc, int op, int want_splits, double* work, hipStream_t stream) {
     if (M < 0 || N < 0 || K < 0 || op < 0 || op > 2 || trans < 0 || trans > 2) {
         mrm_set_error(MRM_BAD_ARG, "dgemm: bad argument (M=%lld N=%lld K=%lld op=%d trans=%d)", (long long)M,
                       (long long)N, (long long)K, op, trans);
@@ -419,13 +424,17 @@ int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A
     }
     const bool vec = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 &&
                      ((lda | ldb) & 15) == 0;
-    const int splits = splitk_count(tm * tn, K);
+    int splits = splitk_count(tm * tn, K);
+    if (want_splits > 0) {
+        const int64_t most = (K + BK - 1) / BK;
+        splits = int(want_splits < most ? want_splits : most);
+    }
     if (splits > 1) {
         const int64_t kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
         const int used = int((K + kchunk - 1) / kchunk);
         const int64_t cstride = M * N;
-        void* W = nullptr;
-        MRM_HIP_TRY(hipMallocAsync(&W, size_t(used) * size_t(cstride) * 8, stream));
+        void* W = work;
+        if (!W) MRM_HIP_TRY(hipMallocAsync(&W, size_t(used) * size_t(cstride) * 8, stream));
         double* w = static_cast<double*>(W);
         switch (trans) {
             case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), stream, used, kchunk, cstride); break;
@@ -435,7 +444,7 @@ int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A
         hipLaunchKernelGGL(splitk_reduce_kernel, dim3(unsigned((M * N + 255) / 256)), dim3(256), 0, stream, w, cstride,
                            used, C, ec, M, N, op);
         MRM_HIP_TRY(hipGetLastError());
-        MRM_HIP_TRY(hipFreeAsync(W, stream));
+        if (!work) MRM_HIP_TRY(hipFreeAsync(W, stream));
         return MRM_OK;
     }
     switch (trans) {
@@ -445,4 +454,17 @@ int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A
     }
     MRM_HIP_TRY(hipGetLastError());
     return MRM_OK;
+}
+
+}  // namespace
+
+int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                     const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream) {
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, nullptr, stream);
+}
+
+int mrm_launch_dgemm_splitk(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                            const double* B, int64_t ldb, double* C, int64_t ldc, int op, int splits,
+                            double* work, hipStream_t stream) {
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, splits < 1 ? 1 : splits, work, stream);
 }

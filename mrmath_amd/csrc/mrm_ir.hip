@@ -68,6 +68,8 @@ struct IrWork {
     size_t perm_elems = 0;
     unsigned long long* norms = nullptr;  // [anorm][bn nrhs][rn nrhs][xn nrhs]
     size_t norm_elems = 0;
+    double* dinv = nullptr;               // diagonal-block inverses of the factors
+    size_t dinv_elems = 0;
 };
 
 template <typename P>
@@ -111,7 +113,10 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
         mrm_set_error(MRM_HIP_ERROR, "lu: panel barrier timed out");
         return MRM_HIP_ERROR;
     }
-    if ((rc = mrm::lu_solve<double>(w.lu, n, n, w.perm, B, ldb, X, ldx, nrhs, s))) return rc;
+    // the factors serve every refinement step: invert their diagonal blocks once
+    double* dinv = mrm::lu_solve_inverse_elems(n) ? w.dinv : nullptr;
+    if (dinv && (rc = mrm::lu_solve_inverses(w.lu, n, n, dinv, s))) return rc;
+    if ((rc = mrm::lu_solve<double>(w.lu, n, n, w.perm, B, ldb, X, ldx, nrhs, s, dinv))) return rc;
 
     unsigned long long* anorm = w.norms;
     unsigned long long* bn = anorm + 1;
@@ -152,7 +157,7 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
         if (berr <= tol) return MRM_OK;
         if (it >= max_iter || !(berr < 0.5 * prev)) return MRM_NOT_CONVERGED;
         prev = berr;
-        if ((rc = mrm::lu_solve<double>(w.lu, n, n, w.perm, w.r, ldr, w.d, ldr, nrhs, s))) return rc;
+        if ((rc = mrm::lu_solve<double>(w.lu, n, n, w.perm, w.r, ldr, w.d, ldr, nrhs, s, dinv))) return rc;
         mrm::add_inplace<double>(X, ldx, w.d, ldr, n, nrhs, s);
     }
 }
@@ -181,6 +186,7 @@ int lineq_solve_ir(int mode, const double* A, int64_t lda, int n, const double* 
     if ((rc = grow(w.indx, w.indx_elems, size_t(n), 4))) return rc;
     if ((rc = grow(w.perm, w.perm_elems, size_t(n), 4))) return rc;
     if ((rc = grow(w.norms, w.norm_elems, size_t(1 + 3 * nrhs), 8))) return rc;
+    if ((rc = grow(w.dinv, w.dinv_elems, mrm::lu_solve_inverse_elems(n), 8))) return rc;
     rc = factor_solve_refine(mode, A, lda, n, B, ldb, X, ldx, nrhs, max_iter, tol, st, s);
     if ((rc == MRM_NOT_CONVERGED || rc == MRM_SINGULAR) && mode != MRM_SOLVE_FP64) {
         // the low-precision factors cannot reach the tolerance for this

@@ -46,15 +46,25 @@ inline int gemm_t<double>(int trans, int64_t M, int64_t N, int64_t K, const doub
 template <typename T>
 int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode = MRM_SOLVE_FP64);
 
+// Factors the m x w panel at P (w <= LU_W) with partial pivoting as one
+// cooperative launch; pivots go to indx[piv_base + j] as absolute rows.
+template <typename T>
+int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm, LuWork& ws, hipStream_t s);
+
 // Packed-operand bytes of the low-precision trailing updates of an n x n LU.
 size_t lu_lowp_bytes(int n, int mode);
 // Grows ws.lowp to lu_lowp_bytes(n, mode); caller holds mrm_host_mutex().
 int lu_lowp_reserve(LuWork& ws, int n, int mode);
 
-// X = (LU)^-1 P B for nrhs columns.
+// X = (LU)^-1 P B for nrhs columns.  dinv (optional): the diagonal-block
+// inverses from lu_solve_inverses for these factors, reused across solves.
 template <typename T>
 int lu_solve(const T* LU, int64_t lda, int n, const int* perm, const T* B, int64_t ldb, T* X, int64_t ldx, int nrhs,
-             hipStream_t s);
+             hipStream_t s, const double* dinv = nullptr);
+// Doubles lu_solve_inverses writes for an n x n factorisation (0 when the
+// solve needs none), and the writer.
+size_t lu_solve_inverse_elems(int n);
+int lu_solve_inverses(const double* LU, int64_t lda, int n, double* dinv, hipStream_t s);
 
 template <typename T>
 void add_inplace(T* X, int64_t ldx, const T* D, int64_t ldd, int rows, int cols, hipStream_t s);

@@ -294,14 +294,32 @@ int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, cons
     int* perm = static_cast<int*>(dP);
     rc = factor_and_status(lu, n, int(n), static_cast<int*>(dI), perm, s);
     if (rc) return rc;
-    if ((rc = mrm::lu_solve<double>(lu, n, int(n), perm, b, m, x, m, int(m), s))) return rc;
-    for (int64_t it = 0; it < NumRefinments; ++it) {
+    // the diagonal-block inverses of the factors serve the solve and every
+    // refinement step
+    void* dinv = nullptr;
+    const size_t inv_elems = mrm::lu_solve_inverse_elems(int(n));
+    if (inv_elems) {
+        MRM_HIP_TRY(hipMallocAsync(&dinv, inv_elems * 8, s));
+        rc = mrm::lu_solve_inverses(lu, n, int(n), static_cast<double*>(dinv), s);
+    }
+    const double* inv = static_cast<const double*>(dinv);
+    if (!rc) rc = mrm::lu_solve<double>(lu, n, int(n), perm, b, m, x, m, int(m), s, inv);
+    for (int64_t it = 0; it < NumRefinments && !rc; ++it) {
         // r = b - A x  (the negated residual of the reference), d = LU^-1 r, x += d
-        MRM_HIP_TRY(hipMemcpyAsync(r, b, size_t(n * m) * 8, hipMemcpyDeviceToDevice, s));
-        if ((rc = mrm_launch_dgemm(MRM_NN, n, m, n, a, n * 8, x, m * 8, r, m * 8, MRM_DO_SUB, s))) return rc;
-        if ((rc = mrm::lu_solve<double>(lu, n, int(n), perm, r, m, d, m, int(m), s))) return rc;
+        if (hipMemcpyAsync(r, b, size_t(n * m) * 8, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+            mrm_set_error(MRM_HIP_ERROR, "lineq solve: residual copy failed");
+            rc = MRM_HIP_ERROR;
+            break;
+        }
+        if ((rc = mrm_launch_dgemm(MRM_NN, n, m, n, a, n * 8, x, m * 8, r, m * 8, MRM_DO_SUB, s))) break;
+        if ((rc = mrm::lu_solve<double>(lu, n, int(n), perm, r, m, d, m, int(m), s, inv))) break;
         mrm::add_inplace<double>(x, m, d, m, int(n), int(m), s);
     }
+    if (dinv) {
+        const hipError_t e = hipFreeAsync(dinv, s);
+        if (!rc && e != hipSuccess) MRM_HIP_TRY(e);
+    }
+    if (rc) return rc;
     MRM_HIP_TRY(hipMemcpy2DAsync(X, size_t(LineWidthX), x, size_t(m) * 8, size_t(m) * 8, size_t(n),
                                  hipMemcpyDeviceToHost, s));
     MRM_HIP_TRY(hipStreamSynchronize(s));

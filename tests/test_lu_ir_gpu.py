@@ -107,6 +107,25 @@ def test_lu_ir_reaches_fp64_backward_error(n, kappa, nrhs, mode):
     assert rel <= kappa * 1e-13, rel
 
 
+@pytest.mark.parametrize("n,boost_from", [(9000, None), (6000, 3000)])
+def test_lu_ir_pivots_from_many_panel_workgroups(n, boost_from):
+    """At these heights the panel kernel spreads its rows over many
+    workgroups, and the pivots come from all of them (with boost_from, rows
+    below it are 100x larger, so every pivot is far from the diagonal).
+    Refinement of an accurate FP32-update factorisation converges in a few
+    steps, without the FP64 fallback."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    if boost_from is not None:
+        A[boost_from:] *= 100.0
+    B = rng.standard_normal((n, 2))
+    rc, X, it, be, fb = _solve_ir(_lib.SOLVE_FP32_IR, A, B)
+    print(f"n={n} boost={boost_from}: rc={rc} steps={it} berr={be:.3e} fell_back={fb}")
+    assert rc == 0 and fb == 0
+    assert it <= 6
+    assert be <= 1e-14 and orc.backward_error(A, X, B) <= 1e-14
+
+
 def test_lu_ir_ill_conditioned_falls_back_or_converges():
     """kappa = 1e10 is beyond what FP32 factors can refine; the solve must still
     return a 1e-14 backward-error solution, through the FP64 fallback if needed."""

@@ -232,22 +232,42 @@ __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const 
         const int e = t - 128;
         st_unit(diag + e, flag, Unit<T>::part(sh.drow[e / UPV], e % UPV));
     }
-    // (c) poll every record (and row j) into LDS
+    // (c) poll every record (and row j) into LDS: a thread's words are read
+    // together, and each round re-reads together the words whose flag is not
+    // there yet, so a poll costs one memory round trip however many words a
+    // thread watches
     const int nrec = G * R::RS;
     const int total = nrec + (own_j ? 0 : R::DU);
     bool late = false;
-    for (int e = t; e < total; e += PANEL_THREADS) {
-        const uint64_t* p = e < nrec ? recs + e : diag + (e - nrec);
-        uint64_t u = ld_unit(p);
+    constexpr int PB = 8;  // words in flight per thread
+    for (int e0 = t; e0 < total; e0 += PB * PANEL_THREADS) {
+        uint64_t u[PB];
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            const int e = e0 + i * PANEL_THREADS;
+            u[i] = e < total ? ld_unit(e < nrec ? recs + e : diag + (e - nrec)) : (uint64_t(flag) << 32);
+        }
         unsigned polls = 0;
-        while (uint32_t(u >> 32) != flag) {
+        for (;;) {
+            bool pending = false;
+#pragma unroll
+            for (int i = 0; i < PB; ++i) pending |= uint32_t(u[i] >> 32) != flag;
+            if (!pending) break;
             if ((++polls & 63u) == 0 && wall_clock64() > deadline) {
                 late = true;
                 break;
             }
-            u = ld_unit(p);
+#pragma unroll
+            for (int i = 0; i < PB; ++i) {
+                const int e = e0 + i * PANEL_THREADS;
+                if (uint32_t(u[i] >> 32) != flag) u[i] = ld_unit(e < nrec ? recs + e : diag + (e - nrec));
+            }
         }
-        sh.raw[e] = uint32_t(u);
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            const int e = e0 + i * PANEL_THREADS;
+            if (e < total) sh.raw[e] = uint32_t(u[i]);
+        }
     }
     if (late) {
         sh.bad = 1;

@@ -1,27 +1,20 @@
-// mrm_lu.hip -- LU factorisation with partial pivoting and triangular solves on gfx950.
+// mrm_lu.hip -- LU factorisation with partial pivoting on gfx950: the driver,
+// row interchanges and the unit-lower triangular blocks.
 //
 // Replaces the reference's recursive LU (InternalRecursiveMatrixLUDecompInPlace,
 // src/LinAlgLU.pas:181-281), its row interchanges (LUSwap, :120-136), the
-// unit-lower triangular solve LUBacksup (:138-169), the Schur update through the
-// blocked multiply (:211-227) and the substitutions of MatrixLUBackSubst
-// (:315-385).
+// unit-lower triangular solve LUBacksup (:138-169) and the Schur update through
+// the blocked multiply (:211-227).
 //
 // Factorisation: right-looking, blocked twice.  Outer blocks of LU_NB = 256
-// columns feed the FP64 MFMA GEMM with K = 256 for the trailing update; each
-// outer block is factored in panels of LU_W = 32 columns by one cooperative
-// kernel that keeps the panel's rows in registers, spread over as many
-// workgroups as the panel height needs; per column every workgroup publishes
-// its candidate in flagged words and polls everybody else's (one memory round
-// trip, no grid barrier) to agree on the pivot.  The pivot rule is the reference's: the first
-// row (in the current row order) of maximal |a| in the column, singular when
-// |pivot| <= 10/MaxDouble; the column is scaled by the reciprocal of the pivot.
-// The pivot sequence is recorded both as indx (row i was exchanged with row
-// indx[i] at step i, the reference's format) and as a permutation vector
-// perm (row i of P*A is row perm[i] of A), which turns the solve's row
-// interchanges into one parallel gather.
-//
-// The arithmetic type is a template parameter: double for the FP64 path, float
-// for the low-precision factorisation of the mixed-precision solver.
+// columns (512 for the low-precision trailing update) feed the GEMM with
+// K = 256 for the trailing update; each outer block is factored in panels of
+// LU_W = 32 columns by the cooperative panel kernel (mrm_lu_panel.hip), with
+// the reference's pivot rule.  The pivot sequence is recorded both as indx
+// (row i was exchanged with row indx[i] at step i, the reference's format)
+// and as a permutation vector perm (row i of P*A is row perm[i] of A), which
+// turns the solve's row interchanges (mrm_lusolve.hip) into one parallel
+// gather.
 
 #include <cfloat>
 #include <climits>
@@ -154,34 +147,6 @@ trsm_lunit_block_kernel(const T* __restrict__ L, int64_t ldl, T* __restrict__ X,
         if (i < rows) X[int64_t(i) * ldx + col] = x[i];
 }
 
-// X = U^-1 X for the R x ncols block, U upper with its diagonal (divides, as
-// MatrixLUBackSubst does, src/LinAlgLU.pas:362-384).
-template <typename T, int R>
-__global__ void __launch_bounds__(128)
-trsm_upper_block_kernel(const T* __restrict__ U, int64_t ldu, T* __restrict__ X, int64_t ldx, int rows, int ncols) {
-    __shared__ T Us[R][R + 1];
-    for (int e = threadIdx.x; e < rows * rows; e += blockDim.x) Us[e / rows][e % rows] = U[int64_t(e / rows) * ldu + e % rows];
-    __syncthreads();
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncols) return;
-    T x[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) x[i] = i < rows ? X[int64_t(i) * ldx + col] : T(0);
-#pragma unroll
-    for (int i = R - 1; i >= 0; --i) {
-        if (i < rows) {
-            T s = x[i];
-#pragma unroll
-            for (int k = i + 1; k < R; ++k)
-                if (k < rows) s = fma(-Us[i][k], x[k], s);
-            x[i] = s / Us[i][i];
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < R; ++i)
-        if (i < rows) X[int64_t(i) * ldx + col] = x[i];
-}
-
 // X += D (rows x cols, element strides)
 template <typename T>
 __global__ void add_inplace_kernel(T* __restrict__ X, int64_t ldx, const T* __restrict__ D, int64_t ldd, int rows,
@@ -230,24 +195,6 @@ int trsm_lower_unit(const T* L, int64_t ldl, T* X, int64_t ldx, int rows, int nc
         if (b + rb < rows) {
             int rc = gemm_t<T>(MRM_NN, rows - b - rb, ncols, rb, L + int64_t(b + rb) * ldl + b, ldl,
                                X + int64_t(b) * ldx, ldx, X + int64_t(b + rb) * ldx, ldx, MRM_DO_SUB, s);
-            if (rc) return rc;
-        }
-    }
-    return MRM_OK;
-}
-
-template <typename T>
-int trsm_upper(const T* U, int64_t ldu, T* X, int64_t ldx, int rows, int ncols, hipStream_t s) {
-    constexpr int R = LU_TRSM_R;
-    // bottom block first
-    int last = ((rows - 1) / R) * R;
-    for (int b = last; b >= 0; b -= R) {
-        const int rb = rows - b < R ? rows - b : R;
-        hipLaunchKernelGGL((trsm_upper_block_kernel<T, R>), dim3(grid1(ncols, 128)), dim3(128), 0, s,
-                           U + int64_t(b) * ldu + b, ldu, X + int64_t(b) * ldx, ldx, rb, ncols);
-        if (b > 0) {
-            // rows above: X[0:b] -= U[0:b, b:b+rb] * X[b:b+rb]
-            int rc = gemm_t<T>(MRM_NN, b, ncols, rb, U + b, ldu, X + int64_t(b) * ldx, ldx, X, ldx, MRM_DO_SUB, s);
             if (rc) return rc;
         }
     }

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "mrm_common.h"
+#include "mrm_wave.h"
 #include "mrm_host.h"
 
 namespace {
@@ -170,12 +171,15 @@ __device__ __forceinline__ void blk_sync() {
 }
 
 // LU with partial pivoting of the n x n matrix L (row stride ld) in LDS by NWV
-// waves; rows are exchanged physically, piv[j] records the row exchanged with
-// j.  Returns false when singular.  wv / wr: NWV slots of LDS scratch.
+// waves; rows are exchanged physically and perm follows them (row i of P*A is
+// row perm[i] of A).  Returns false when singular.  wv / wr: NWV slots of LDS
+// scratch.
 template <typename T, int NWV>
-__device__ bool block_lu(T* L, int ld, int n, int* piv, int tid, T* wv, int* wr) {
+__device__ bool block_lu(T* L, int ld, int n, int* perm, int tid, T* wv, int* wr) {
     constexpr int NT = 64 * NWV;
     const int lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < n; i += NT) perm[i] = i;
+    blk_sync<NWV>();
     for (int j = 0; j < n; ++j) {
         // first row of maximal |a| in column j
         T best = T(-1);
@@ -184,12 +188,7 @@ __device__ bool block_lu(T* L, int ld, int n, int* piv, int tid, T* wv, int* wr)
             const T v = fabs(L[i * ld + j]);
             if (v > best) { best = v; brow = i; }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(best, off, 64);
-            const int orow = __shfl_xor(brow, off, 64);
-            if (ov > best || (ov == best && orow < brow)) { best = ov; brow = orow; }
-        }
+        mrm::wave::argmax(best, brow);
         if constexpr (NWV > 1) {
             if (lane == 0) {
                 wv[wave] = best;
@@ -202,8 +201,12 @@ __device__ bool block_lu(T* L, int ld, int n, int* piv, int tid, T* wv, int* wr)
             for (int q = 1; q < NWV; ++q)
                 if (wv[q] > best || (wv[q] == best && wr[q] < brow)) { best = wv[q]; brow = wr[q]; }
         }
-        if (tid == 0) piv[j] = brow;
         if (!(best > tiny_pivot<T>())) return false;  // uniform
+        if (brow != j && tid == 0) {
+            const int pj = perm[j];
+            perm[j] = perm[brow];
+            perm[brow] = pj;
+        }
         if (brow != j)
             for (int k = tid; k < n; k += NT) {
                 const T t = L[j * ld + k];
@@ -237,14 +240,15 @@ __device__ bool block_lu(T* L, int ld, int n, int* piv, int tid, T* wv, int* wr)
     return true;
 }
 
-// y := (LU)^-1 P y in place (y in LDS, type Y), with the factors of wave_lu.
+// y := (LU)^-1 P y in place (y in LDS, type Y, n <= 128), with the factors
+// and permutation of block_lu.  P y is one gather.
 template <typename T, typename Y>
-__device__ void wave_lu_subst(const T* L, int ld, int n, const int* piv, Y* y, int lane) {
-    if (lane == 0)
-        for (int j = 0; j < n; ++j) {
-            const int r = piv[j];
-            if (r != j) { const Y t = y[j]; y[j] = y[r]; y[r] = t; }
-        }
+__device__ void wave_lu_subst(const T* L, int ld, int n, const int* perm, Y* y, int lane) {
+    const Y g0 = lane < n ? y[perm[lane]] : Y(0);
+    const Y g1 = lane + 64 < n ? y[perm[lane + 64]] : Y(0);
+    wave_sync();
+    if (lane < n) y[lane] = g0;
+    if (lane + 64 < n) y[lane + 64] = g1;
     wave_sync();
     for (int k = 0; k < n; ++k) {            // unit lower
         const Y yk = y[k];
@@ -278,7 +282,7 @@ __global__ void __launch_bounds__(64 * NWV) bl_solve_kernel(const LuP* __restric
     T* L = reinterpret_cast<T*>(lds);
     double* xv = reinterpret_cast<double*>(lds + ((size_t(nmax) * (nmax + 1) * sizeof(T) + 15) & ~size_t(15)));
     double* rv = xv + nmax;
-    int* piv = reinterpret_cast<int*>(rv + nmax);
+    int* perm = reinterpret_cast<int*>(rv + nmax);
     for (;;) {
         if (tid == 0) s_item = int(atomicAdd(counter, 1u));
         __syncthreads();
@@ -308,17 +312,17 @@ __global__ void __launch_bounds__(64 * NWV) bl_solve_kernel(const LuP* __restric
         }
         for (int i = tid; i < n; i += NT) xv[i] = s.b[i];
         __syncthreads();
-        const bool ok = block_lu<T, NWV>(L, ld, n, piv, tid, wv, wr);
+        const bool ok = block_lu<T, NWV>(L, ld, n, perm, tid, wv, wr);
         if (ok) {
             if (MODE == 0) {
-                if (wave == 0) wave_lu_subst<T, double>(L, ld, n, piv, xv, lane);
+                if (wave == 0) wave_lu_subst<T, double>(L, ld, n, perm, xv, lane);
                 __syncthreads();
             } else {
                 // FP32 solve of the FP64 right-hand side, then FP64 refinement
                 float* yv = reinterpret_cast<float*>(rv);
                 for (int i = tid; i < n; i += NT) yv[i] = float(xv[i]);
                 __syncthreads();
-                if (wave == 0) wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
+                if (wave == 0) wave_lu_subst<T, float>(L, ld, n, perm, yv, lane);
                 __syncthreads();
                 for (int i = tid; i < n; i += NT) xv[i] = double(yv[i]);
                 __syncthreads();
@@ -331,7 +335,7 @@ __global__ void __launch_bounds__(64 * NWV) bl_solve_kernel(const LuP* __restric
                         yv[i] = float(r);  // threads own disjoint rows
                     }
                     __syncthreads();
-                    if (wave == 0) wave_lu_subst<T, float>(L, ld, n, piv, yv, lane);
+                    if (wave == 0) wave_lu_subst<T, float>(L, ld, n, perm, yv, lane);
                     __syncthreads();
                     for (int i = tid; i < n; i += NT) xv[i] += double(yv[i]);
                     __syncthreads();

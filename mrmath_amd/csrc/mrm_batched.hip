@@ -78,16 +78,22 @@ __global__ void __launch_bounds__(BG_THREADS) bg_gemm_kernel(const GemmP* __rest
         const GemmP pr = probs[tl.p];
         const int64_t m0 = tl.m0, n0 = tl.n0;
         double ra[EA], rb[EA];
+        // element tid + 256 i of a chunk: A row ar + (256/BK) i, column ak
+        // (k fastest, row-major A); B k-row bk + (256/T) i, column bc
+        static_assert(BG_THREADS % BG_BK == 0 && BG_THREADS % T == 0, "chunk layout");
+        const int ak = tid % BG_BK, ar = tid / BG_BK;
+        const int bc = tid % T, bk = tid / T;
+        const double* pa = pr.A + (m0 + ar) * pr.lda + ak;
+        const double* pb = pr.B + int64_t(bk) * pr.ldb + n0 + bc;
+        const bool bcol_ok = n0 + bc < pr.N;
         auto fetch = [&](int64_t k0) {
+            const bool ak_ok = k0 + ak < pr.K;
 #pragma unroll
             for (int i = 0; i < EA; ++i) {
-                const int idx = tid + BG_THREADS * i;
-                const int k = idx % BG_BK, r = idx / BG_BK;  // A: k fastest (row-major A)
-                const int64_t gr = m0 + r, gk = k0 + k;
-                ra[i] = (gr < pr.M && gk < pr.K) ? pr.A[gr * pr.lda + gk] : 0.0;
-                const int c = idx % T, kb = idx / T;        // B: column fastest
-                const int64_t gc = n0 + c, gkb = k0 + kb;
-                rb[i] = (gc < pr.N && gkb < pr.K) ? pr.B[gkb * pr.ldb + gc] : 0.0;
+                const int rr = ar + (BG_THREADS / BG_BK) * i;
+                ra[i] = (ak_ok && m0 + rr < pr.M) ? pa[int64_t((BG_THREADS / BG_BK) * i) * pr.lda + k0] : 0.0;
+                const int kr = bk + (BG_THREADS / T) * i;
+                rb[i] = (bcol_ok && k0 + kr < pr.K) ? pb[(k0 + (BG_THREADS / T) * i) * pr.ldb] : 0.0;
             }
         };
         auto stash = [&]() {

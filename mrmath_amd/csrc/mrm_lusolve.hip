@@ -21,7 +21,8 @@
 namespace {
 
 constexpr int TR = 128;  // diagonal block size
-constexpr int SOLVE_SPLITS = 4;  // K chunks of the block products
+constexpr int SOLVE_SPLITS = 4;      // K chunks of the block products for few right-hand sides
+constexpr int SOLVE_WIDE_RHS = 256;  // from this many right-hand sides the products fill the GPU unsplit
 
 __global__ void __launch_bounds__(256) gather_kernel(const double* __restrict__ B, int64_t ldb,
                                                      double* __restrict__ Y, int64_t ldy, const int* __restrict__ perm,
@@ -133,11 +134,13 @@ int lu_solve(const T* LU, int64_t lda, int n, const int* perm, const T* B, int64
     void* mem = nullptr;
     // Y, the partial products of the split products, and the inverses unless given
     const size_t rhs = size_t(n) * size_t(nrhs);
-    MRM_HIP_TRY(hipMallocAsync(&mem, ((1 + SOLVE_SPLITS) * rhs + (dinv ? 0 : inv_elems)) * sizeof(double), s));
+    const int splits = nrhs < SOLVE_WIDE_RHS ? SOLVE_SPLITS : 1;
+    const size_t part_elems = splits > 1 ? size_t(splits) * rhs : 0;
+    MRM_HIP_TRY(hipMallocAsync(&mem, (rhs + part_elems + (dinv ? 0 : inv_elems)) * sizeof(double), s));
     double* Y = static_cast<double*>(mem);
     double* part = Y + rhs;
     if (!dinv) {
-        double* own = part + SOLVE_SPLITS * rhs;
+        double* own = part + part_elems;
         int rc = lu_solve_inverses(LU, lda, n, own, s);
         if (rc) {
             (void)hipFreeAsync(mem, s);
@@ -150,13 +153,13 @@ int lu_solve(const T* LU, int64_t lda, int n, const int* perm, const T* B, int64
     hipLaunchKernelGGL(gather_kernel, dim3(unsigned((int64_t(n) * nrhs + 255) / 256)), dim3(256), 0, s, B, ldb, X,
                        ldx, perm, n, nrhs);
     int rc = MRM_OK;
-    // K of every product is one diagonal block (<= 128): cut it into
-    // SOLVE_SPLITS chunks so that the skinny products occupy that many times
-    // the workgroups
+    // K of every product is one diagonal block (<= 128): with few right-hand
+    // sides, cut it into SOLVE_SPLITS chunks so that the skinny products
+    // occupy that many times the workgroups
     auto mm = [&](int64_t M, int64_t K, const double* Ap, int64_t lda_, const double* Bp, int64_t ldb_, double* Cp,
                   int64_t ldc_, int op) {
         return mrm_launch_dgemm_splitk(MRM_NN, M, nrhs, K, Ap, lda_ * 8, Bp, ldb_ * 8, Cp, ldc_ * 8, op,
-                                       SOLVE_SPLITS, part, s);
+                                       splits, splits > 1 ? part : nullptr, s);
     };
     for (int b = 0; b < nb && !rc; ++b) {
         const int r0 = b * TR, rb = n - r0 < TR ? n - r0 : TR;

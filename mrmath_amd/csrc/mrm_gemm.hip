@@ -20,8 +20,9 @@
 //
 // LDS panel layouts (r = the panel's M or N index):
 //   KC (operand K-contiguous in HBM): [r][k], 16 doubles per row, the eight
-//       16-byte chunks of row r stored at slot (chunk ^ ((r/2) % 8)) so that
-//       the fragment reads of 16 consecutive rows hit distinct banks;
+//       16-byte chunks of row r stored at slot (chunk ^ (r % 8)), so that the
+//       four rows of a B fragment read hit distinct banks (a row is exactly
+//       one pass of the banks; an A fragment read of 16 rows is two-way);
 //   RC (operand r-contiguous in HBM): [k][r], row stride SR = 132 doubles.
 // One direct-load instruction of a wave writes 1 KiB of LDS (64 lanes x 16 B):
 // eight KC rows, or one RC k-row.
@@ -68,7 +69,7 @@ constexpr int PANEL = KC_PANEL > RC_PANEL ? KC_PANEL : RC_PANEL;   // doubles pe
 typedef const __attribute__((address_space(1))) void* gptr_t;
 typedef __attribute__((address_space(3))) void* lptr_t;
 
-__device__ __forceinline__ int kc_swizzle(int r) { return (r >> 1) & 7; }
+__device__ __forceinline__ int kc_swizzle(int r) { return r & 7; }
 
 // LDS element index of (r, k) in a panel.
 template <bool KC>
@@ -79,7 +80,7 @@ __device__ __forceinline__ int panel_index(int r, int k) {
 
 // Moves one 128 x 16 panel per K step.  Wave w, instruction i, lane L:
 //   KC: panel row r = 32w + 8i + L/8, LDS slot L%8 <- HBM chunk (L%8) ^ swz(r);
-//       swz(r + 8) = swz(r) ^ 4, so odd instructions read chunk offset ^ 4
+//       swz(r + 8) = swz(r), so every instruction reads the same chunk offset
 //   RC: panel k-row 4w + i, rows r = 2L, 2L+1
 // base points at the operand's element (0, 0); ld in elements.
 template <bool KC, bool VEC>
@@ -92,8 +93,7 @@ struct PanelMover {
     int lds0;            // LDS element of this lane's 16 bytes, instruction 0
     int lds_wave0;       // LDS element where this wave's instruction 0 writes (uniform)
     int ldsstep;         // LDS elements between instructions
-    int koff;            // KC: this lane's first k (even instructions); RC: this wave's first k-row
-    int kodd;            // KC: element shift of odd instructions' k (koff ^ 8 - koff)
+    int koff;            // KC: this lane's first k; RC: this wave's first k-row
     bool rows_full;      // all 128 panel rows exist (uniform)
     unsigned rmask;      // KC: bit i = row of instruction i exists; RC: bit0/1 = row pair exists
     double2 v[4];        // register path staging
@@ -108,7 +108,6 @@ struct PanelMover {
             const int r = 32 * w + (L >> 3);
             const int slot = L & 7;
             koff = 2 * (slot ^ kc_swizzle(r));
-            kodd = (koff ^ 8) - koff;
             p = base + (r0 + r) * ld + koff;
             istep = 8 * ld;
             kstep = BK;
@@ -120,7 +119,6 @@ struct PanelMover {
             for (int i = 0; i < 4; ++i) rmask |= (r0 + r + 8 * i < R) ? (1u << i) : 0u;
         } else {
             koff = 4 * w;
-            kodd = 0;
             p = base + int64_t(koff) * ld + r0 + 2 * L;
             istep = ld;
             kstep = int64_t(BK) * ld;
@@ -137,7 +135,7 @@ struct PanelMover {
     __device__ __forceinline__ void issue_direct(double* __restrict__ P) const {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            __builtin_amdgcn_global_load_lds((gptr_t)(p + i * istep + (i & 1) * kodd),
+            __builtin_amdgcn_global_load_lds((gptr_t)(p + i * istep),
                                              (lptr_t)(P + lds_wave0 + i * ldsstep), 16, 0, 0);
     }
 
@@ -145,11 +143,11 @@ struct PanelMover {
     __device__ __forceinline__ void load_regs() {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const double* q = p + i * istep + (i & 1) * kodd;
+            const double* q = p + i * istep;
             bool ok0, ok1;
             if (KC) {
                 const bool rok = (rmask >> i) & 1u;
-                const int64_t k = k0 + koff + (i & 1) * kodd;
+                const int64_t k = k0 + koff;
                 ok0 = rok && (k < K);
                 ok1 = rok && (k + 1 < K);
             } else {

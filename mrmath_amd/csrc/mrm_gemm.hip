@@ -21,8 +21,7 @@
 // LDS panel layouts (r = the panel's M or N index):
 //   KC (operand K-contiguous in HBM): [r][k], 16 doubles per row, the eight
 //       16-byte chunks of row r stored at slot (chunk ^ (r % 8)), so that the
-//       four rows of a B fragment read hit distinct banks (a row is exactly
-//       one pass of the banks; an A fragment read of 16 rows is two-way);
+//       16-byte fragment reads of eight consecutive rows hit distinct banks;
 //   RC (operand r-contiguous in HBM): [k][r], row stride SR = 132 doubles.
 // One direct-load instruction of a wave writes 1 KiB of LDS (64 lanes x 16 B):
 // eight KC rows, or one RC k-row.
@@ -43,15 +42,15 @@
 // quads cover the sub-tile.  The accumulator acc[p][c] element q therefore
 // holds row 16p + 4((l/4)%4) + l/16, column 16c + 4q + l%4.
 //
-// K ordering inside a step: the 16 k of a step are visited as 4 rounds kk of
-// 4, lane group g = l/16 taking k = g + 4kk in round kk, for A and B alike, so
-// every product A[m,k]*B[k,n] is formed exactly once (and no two rounds read
-// neighbouring doubles of a KC row, which would invite the compiler to merge
-// the reads and hold them longer).
+// K ordering inside a step: the MFMA's k index is the lane group g = l/16; a
+// step is two halves m, and in half m the first MFMA of a pair takes
+// k = 8m + 2g, the second k = 8m + 2g + 1, for A and B alike, so every
+// product A[m,k]*B[k,n] is formed exactly once and a lane's two k of a KC row
+// are one 16-byte chunk (see frag_pair).
 //
-// Pipelining: one round kk and one sub-column c form a group of 16 MFMAs
-// (256 cycles); the fragments of the next group are read from LDS while the
-// current group issues.
+// Pipelining: one half step m and one sub-column c form a group of 32 MFMAs;
+// the fragments of the next group are read from LDS while the current group
+// issues.
 
 #include "mrm_common.h"
 
@@ -76,6 +75,20 @@ template <bool KC>
 __device__ __forceinline__ int panel_index(int r, int k) {
     if (KC) return r * BK + 2 * ((k >> 1) ^ kc_swizzle(r)) + (k & 1);
     return k * SR + r;
+}
+
+// The fragments of one K step are read as pairs: half step m (0, 1), lane
+// group g = lane/16 takes k = 8m + 2g (.x) and 8m + 2g + 1 (.y), for A and B
+// alike, so every product A[r,k]*B[k,n] is formed exactly once.  In a KC panel
+// the pair is one 16-byte chunk (one ds_read_b128; eight consecutive rows hit
+// distinct banks), in an RC panel two k-rows.
+template <bool KC>
+__device__ __forceinline__ double2 frag_pair(const double* __restrict__ P, int r, int m, int g) {
+    if (KC) return *reinterpret_cast<const double2*>(P + r * BK + 2 * ((4 * m + g) ^ kc_swizzle(r)));
+    double2 v;
+    v.x = P[(8 * m + 2 * g) * SR + r];
+    v.y = P[(8 * m + 2 * g + 1) * SR + r];
+    return v;
 }
 
 // Moves one 128 x 16 panel per K step.  Wave w, instruction i, lane L:
@@ -256,7 +269,6 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     // fragment rows of this lane: A row wm*64 + 16p + li, B column wn*64 + 16c + 4q + (lane & 3)
     const int arow = wm * 64 + li;
     const int bcol = wn * 64 + (lane & 3);
-    const int kq = lg;   // round kk reads k = lg + 4kk
 
     for (int64_t t = 0; t < nk; ++t) {
         const int cur = int(t & 1);
@@ -275,28 +287,33 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         }
         const double* As = smem + cur * 2 * PANEL;
         const double* Bs = As + PANEL;
-        double a_cur[4], a_nxt[4], b_cur[4], b_nxt[4];
+        double2 a_cur[4], a_nxt[4], b_cur[4], b_nxt[4];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) a_cur[p] = As[panel_index<AKC>(arow + 16 * p, kq)];
+        for (int p = 0; p < 4; ++p) a_cur[p] = frag_pair<AKC>(As, arow + 16 * p, 0, lg);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) b_cur[q] = Bs[panel_index<BKC>(bcol + 4 * q, kq)];   // K step 0
+        for (int q = 0; q < 4; ++q) b_cur[q] = frag_pair<BKC>(Bs, bcol + 4 * q, 0, lg);
 #pragma unroll
-        for (int gidx = 0; gidx < 16; ++gidx) {
+        for (int gidx = 0; gidx < 8; ++gidx) {
             const int c = gidx & 3;
-            if (gidx + 1 < 16) {
-                const int kk2 = (gidx + 1) >> 2, c2 = (gidx + 1) & 3;
+            if (gidx + 1 < 8) {
+                const int m2 = (gidx + 1) >> 2, c2 = (gidx + 1) & 3;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) b_nxt[q] = Bs[panel_index<BKC>(bcol + 16 * c2 + 4 * q, kq + 4 * kk2)];
+                for (int q = 0; q < 4; ++q) b_nxt[q] = frag_pair<BKC>(Bs, bcol + 16 * c2 + 4 * q, m2, lg);
                 if (c2 == 0) {
 #pragma unroll
-                    for (int p = 0; p < 4; ++p) a_nxt[p] = As[panel_index<AKC>(arow + 16 * p, kq + 4 * kk2)];
+                    for (int p = 0; p < 4; ++p) a_nxt[p] = frag_pair<AKC>(As, arow + 16 * p, m2, lg);
                 }
             }
 #pragma unroll
             for (int p = 0; p < 4; ++p)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    acc[p][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a_cur[p], b_cur[q], acc[p][c][q], 0, 0, 0);
+                    acc[p][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a_cur[p].x, b_cur[q].x, acc[p][c][q], 0, 0, 0);
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc[p][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a_cur[p].y, b_cur[q].y, acc[p][c][q], 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) b_cur[q] = b_nxt[q];
             if (((gidx + 1) & 3) == 0) {

@@ -404,6 +404,13 @@ def main():
             if w.startswith("syrk"):
                 rows[w]["hbm_gbps_algorithmic"] = round((q["M"] * q["K"] + q["M"] * q["N"]) * 8 /
                                                         (q["ms_per_step"] * 1e-3) / 1e9, 1)
+                # X^T X is symmetric: the library computes the 128 x 128 tiles on
+                # and above the diagonal and mirrors them; tflops counts the full
+                # product 2 M N K, executed_tflops what the matrix cores ran
+                t = -(-q["M"] // 128)
+                frac = t * (t + 1) / 2 / (t * t)
+                rows[w]["symmetric_tiles"] = f"{t * (t + 1) // 2}/{t * t}"
+                rows[w]["executed_tflops"] = round(q["tflops"] * frac, 3)
         rows["batched"] = batched_row(lib)
         rows["ozaki_8192"] = ozaki_row(lib)
         rows["lu_ir_16384"] = lu_ir_row(lib)

@@ -192,13 +192,25 @@ __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" 
 // XCDs; remap so that each XCD receives a contiguous run of logical tiles, then
 // walk that run in groups of GROUP_M tile-rows so the workgroups resident on
 // one XCD share A and B panels in that XCD's L2.
-__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
+// upper: only the tiles with tm <= tn of a square tile grid (a symmetric
+// product), numbered row by row.
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int upper, int& tm, int& tn) {
     constexpr int NXCD = 8;
     constexpr int GROUP_M = 8;
-    const int total = tiles_m * tiles_n;
+    const int total = upper ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
     const int xcd = bid % NXCD;
     const int q = total / NXCD, rem = total % NXCD;
     const int logical = xcd * q + (xcd < rem ? xcd : rem) + bid / NXCD;
+    if (upper) {
+        int r = logical, row = 0;
+        while (r >= tiles_m - row) {
+            r -= tiles_m - row;
+            ++row;
+        }
+        tm = row;
+        tn = row + r;
+        return;
+    }
     const int per_group = GROUP_M * tiles_n;
     const int group = logical / per_group;
     const int first_m = group * GROUP_M;
@@ -213,7 +225,7 @@ template <bool TA, bool TB, bool VEC>
 __global__ void __launch_bounds__(NTHREADS, 2)
 dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A, int64_t lda,
                   const double* __restrict__ B, int64_t ldb, double* __restrict__ C, int64_t ldc,
-                  int op, int tiles_m, int tiles_n, int64_t kchunk, int64_t cstride) {
+                  int op, int tiles_m, int tiles_n, int64_t kchunk, int64_t cstride, int upper) {
     __shared__ __attribute__((aligned(16))) double smem[2 * 2 * PANEL];
     if (gridDim.y > 1) {
         // split K: chunk blockIdx.y of kchunk (a multiple of BK) writes its
@@ -234,7 +246,7 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     const int li = lane & 15, lg = lane >> 4;
 
     int tm, tn;
-    tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+    tile_coords(blockIdx.x, tiles_m, tiles_n, upper, tm, tn);
     const int64_t m0 = int64_t(tm) * BM;
     const int64_t n0 = int64_t(tn) * BN;
 
@@ -352,6 +364,13 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
                     if (op == MRM_DO_NONE) crow[col] = x;
                     else if (op == MRM_DO_ADD) crow[col] = crow[col] + x;
                     else crow[col] = crow[col] - x;
+                    if (upper && tm != tn) {
+                        // the mirrored element of a symmetric product (M == N)
+                        double* cm = C + col * ldc + row;
+                        if (op == MRM_DO_NONE) *cm = x;
+                        else if (op == MRM_DO_ADD) *cm = *cm + x;
+                        else *cm = *cm - x;
+                    }
                 }
             }
         }
@@ -360,15 +379,16 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
 
 template <bool TA, bool TB>
 void launch_variant(bool vec, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
-                    int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, hipStream_t s,
+                    int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, int upper, hipStream_t s,
                     int splits = 1, int64_t kchunk = 0, int64_t cstride = 0) {
-    dim3 grid(unsigned(tiles_m) * unsigned(tiles_n), unsigned(splits)), block(NTHREADS);
+    const unsigned ntiles = upper ? unsigned(tiles_m) * unsigned(tiles_m + 1) / 2 : unsigned(tiles_m) * unsigned(tiles_n);
+    dim3 grid(ntiles, unsigned(splits)), block(NTHREADS);
     if (vec)
         hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n, kchunk, cstride);
+                           op, tiles_m, tiles_n, kchunk, cstride, upper);
     else
         hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n, kchunk, cstride);
+                           op, tiles_m, tiles_n, kchunk, cstride, upper);
 }
 
 // C (op)= sum_c W[c] over `splits` partial products, in chunk order.
@@ -390,7 +410,9 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(const double* __rest
 // (the covariance shape X^T X with X tall): up to 64 chunks of at least 512.
 int splitk_count(int64_t tiles, int64_t K) {
     if (tiles >= 128 || K < 4096) return 1;
-    int64_t p = (512 + tiles - 1) / tiles;
+    // at most one wave of workgroups (two per CU): a partial second wave
+    // would double the time
+    int64_t p = 512 / tiles;
     p = p < K / 512 ? p : K / 512;
     return int(p < 64 ? p : 64);
 }
@@ -439,7 +461,12 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
     }
     const bool vec = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 &&
                      ((lda | ldb) & 15) == 0;
-    int splits = splitk_count(tm * tn, K);
+    // A^T A and A A^T are symmetric: compute the tiles on and above the
+    // diagonal and mirror them (the mirrored sums are the same products in the
+    // same order, so the result is the full product's bit for bit)
+    const int upper = (trans != MRM_NN && A == B && lda == ldb && M == N) ? 1 : 0;
+    const int64_t ntiles = upper ? tm * (tm + 1) / 2 : tm * tn;
+    int splits = splitk_count(ntiles, K);
     if (want_splits > 0) {
         const int64_t most = (K + BK - 1) / BK;
         splits = int(want_splits < most ? want_splits : most);
@@ -452,9 +479,9 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
         if (!W) MRM_HIP_TRY(hipMallocAsync(&W, size_t(used) * size_t(cstride) * 8, stream));
         double* w = static_cast<double*>(W);
         switch (trans) {
-            case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), stream, used, kchunk, cstride); break;
-            case MRM_TN: launch_variant<true, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), stream, used, kchunk, cstride); break;
-            default:     launch_variant<false, true>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), stream, used, kchunk, cstride); break;
+            case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), upper, stream, used, kchunk, cstride); break;
+            case MRM_TN: launch_variant<true, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), upper, stream, used, kchunk, cstride); break;
+            default:     launch_variant<false, true>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), upper, stream, used, kchunk, cstride); break;
         }
         hipLaunchKernelGGL(splitk_reduce_kernel, dim3(unsigned((M * N + 255) / 256)), dim3(256), 0, stream, w, cstride,
                            used, C, ec, M, N, op);
@@ -463,9 +490,9 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
         return MRM_OK;
     }
     switch (trans) {
-        case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), stream); break;
-        case MRM_TN: launch_variant<true, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), stream); break;
-        default:     launch_variant<false, true>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), stream); break;
+        case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), upper, stream); break;
+        case MRM_TN: launch_variant<true, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), upper, stream); break;
+        default:     launch_variant<false, true>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), upper, stream); break;
     }
     MRM_HIP_TRY(hipGetLastError());
     return MRM_OK;

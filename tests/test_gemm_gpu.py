@@ -268,3 +268,29 @@ def test_covariance_shape_split_k():
     ref_rows = orc.generic_transp_mult(np.ascontiguousarray(Xn[:, cols]), Xn)
     err = np.linalg.norm(Cn[cols] - ref_rows) / (np.linalg.norm(Xn[:, cols]) * np.linalg.norm(Xn))
     assert err <= 4 * k * U, err
+
+
+@pytest.mark.parametrize("trans", [_lib.TN, _lib.NT])
+@pytest.mark.parametrize("n,k,op", [(300, 1000, 0), (300, 1000, 2), (200, 50000, 1), (129, 40000, 0)])
+def test_symmetric_product_equals_general_path(trans, n, k, op):
+    """A^T A (TN) and A A^T (NT) with the same operand for both sides compute
+    only the tiles on and above the diagonal and mirror them.  The mirrored
+    sums are the same products in the same order, so the result must equal,
+    bit for bit, the general path run on a copy of the operand (a different
+    pointer), for every destination operation, with and without split K."""
+    import torch
+    lib = _lib.load()
+    g = torch.Generator(device="cuda").manual_seed(n + k + op)
+    shape = (k, n) if trans == _lib.TN else (n, k)
+    X = torch.rand(*shape, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    Y = X.clone()
+    C0 = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g)
+    C1, C2 = C0.clone(), C0.clone()
+    ld = shape[1] * 8
+    s_ = torch.cuda.current_stream().cuda_stream
+    assert lib.mrm_dev_dgemm(trans, n, n, k, X.data_ptr(), ld, X.data_ptr(), ld, C1.data_ptr(), n * 8, op, s_) == 0
+    assert lib.mrm_dev_dgemm(trans, n, n, k, X.data_ptr(), ld, Y.data_ptr(), ld, C2.data_ptr(), n * 8, op, s_) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    if op == 0:
+        assert torch.equal(C1, C1.T)

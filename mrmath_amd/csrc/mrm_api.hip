@@ -142,11 +142,15 @@ static int host_gemm(int trans, double* dest, int64_t destLW, const double* mt1,
     std::lock_guard<std::mutex> lk(mrm_host_mutex());
     hipStream_t s = mrm_stream();
     void *dA, *dB, *dC;
+    // one operand used twice (X^T X, X X^T) is staged once, which also lets
+    // the device product take its symmetric path
+    const bool same = mt1 == mt2 && LW1 == LW2 && w1 == w2 && h1 == h2;
     if ((rc = mrm_staging(0, size_t(w1 * h1) * 8, &dA))) return rc;
-    if ((rc = mrm_staging(1, size_t(w2 * h2) * 8, &dB))) return rc;
+    if (!same && (rc = mrm_staging(1, size_t(w2 * h2) * 8, &dB))) return rc;
     if ((rc = mrm_staging(2, size_t(M * N) * 8, &dC))) return rc;
     if ((rc = copy_in(dA, mt1, LW1, w1, h1, s))) return rc;
-    if ((rc = copy_in(dB, mt2, LW2, w2, h2, s))) return rc;
+    if (same) dB = dA;
+    else if ((rc = copy_in(dB, mt2, LW2, w2, h2, s))) return rc;
     if (op != MRM_DO_NONE && (rc = copy_in(dC, dest, destLW, N, M, s))) return rc;
     if (slices == 0) {
         rc = mrm_launch_dgemm(trans, M, N, K, static_cast<const double*>(dA), w1 * 8, static_cast<const double*>(dB),

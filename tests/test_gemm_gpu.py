@@ -294,3 +294,18 @@ def test_symmetric_product_equals_general_path(trans, n, k, op):
     assert torch.equal(C1, C2)
     if op == 0:
         assert torch.equal(C1, C1.T)
+
+
+def test_symmetric_host_entry_stages_once():
+    """The host MultT1 / MultT2 entries with the same matrix twice take the
+    symmetric device path; the result equals the one for a copy of the
+    operand bit for bit."""
+    X = orc.pm1_matrix(300, 2000, 77)
+    for mult in (ss.MatrixMultT1, ss.MatrixMultT2):
+        Y = X if mult is ss.MatrixMultT1 else np.ascontiguousarray(X.T)
+        n = Y.shape[1] if mult is ss.MatrixMultT1 else Y.shape[0]
+        C1, C2 = np.zeros((n, n)), np.zeros((n, n))
+        mult(C1, Y, Y)
+        mult(C2, Y, Y.copy())
+        assert np.array_equal(C1, C2)
+        assert np.array_equal(C1, C1.T)

@@ -334,7 +334,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=None,
                     choices=sorted(WORKLOADS) + ["ozaki_8192", "lu_ir_16384", "batched", "copy_1gib"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -397,7 +397,9 @@ def main():
     if args.workload is None and not args.no_rows:
         rows = {}
         for w in ("dgemm_tn_4096", "dgemm_nt_4096", "syrk_200000x512"):
-            q = gemm_run(lib, w, 10, 2)
+            # ten untimed launches first: the first tens of milliseconds of
+            # GEMM work after a pause run at a lower clock
+            q = gemm_run(lib, w, 10, 10)
             rows[w] = {"tflops": round(q["tflops"], 3), "ms": round(q["ms_per_step"], 4),
                        "pct_of_fp64_peak": round(100 * q["tflops"] / FP64_PEAK_TFLOPS, 2),
                        "linearity_rel_err": q["linearity_rel_err"]}

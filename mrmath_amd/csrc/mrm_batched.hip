@@ -63,8 +63,8 @@ __global__ void __launch_bounds__(BG_THREADS) bg_gemm_kernel(const GemmP* __rest
     constexpr int BP = T / 32;               // 16x16 blocks per wave per dimension
     constexpr int EA = T * BG_BK / BG_THREADS;  // elements per thread per chunk
     constexpr int LDT = T + 2;
-    __shared__ double As[BG_BK][LDT];
-    __shared__ double Bs[BG_BK][LDT];
+    __shared__ double As[2][BG_BK][LDT];  // two stages: one barrier per K chunk
+    __shared__ double Bs[2][BG_BK][LDT];
     __shared__ int s_item;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -96,12 +96,12 @@ __global__ void __launch_bounds__(BG_THREADS) bg_gemm_kernel(const GemmP* __rest
                 rb[i] = (bcol_ok && k0 + kr < pr.K) ? pb[(k0 + (BG_THREADS / T) * i) * pr.ldb] : 0.0;
             }
         };
-        auto stash = [&]() {
+        auto stash = [&](int st) {
 #pragma unroll
             for (int i = 0; i < EA; ++i) {
                 const int idx = tid + BG_THREADS * i;
-                As[idx % BG_BK][idx / BG_BK] = ra[i];
-                Bs[idx / T][idx % T] = rb[i];
+                As[st][idx % BG_BK][idx / BG_BK] = ra[i];
+                Bs[st][idx / T][idx % T] = rb[i];
             }
         };
         mrm_d4 acc[BP][BP];
@@ -110,25 +110,31 @@ __global__ void __launch_bounds__(BG_THREADS) bg_gemm_kernel(const GemmP* __rest
 #pragma unroll
             for (int c = 0; c < BP; ++c) acc[p][c] = mrm_d4{0.0, 0.0, 0.0, 0.0};
         fetch(0);
+        stash(0);
+        __syncthreads();
+        int st = 0;
         for (int64_t k0 = 0; k0 < pr.K; k0 += BG_BK) {
-            stash();
-            __syncthreads();
-            if (k0 + BG_BK < pr.K) fetch(k0 + BG_BK);
+            const bool more = k0 + BG_BK < pr.K;
+            if (more) fetch(k0 + BG_BK);
 #pragma unroll
             for (int kk = 0; kk < BG_BK / 4; ++kk) {
                 const int k = 4 * kk + (lane >> 4);
                 double a[BP], b[BP];
 #pragma unroll
-                for (int p = 0; p < BP; ++p) a[p] = As[k][wm * (T / 2) + 16 * p + (lane & 15)];
+                for (int p = 0; p < BP; ++p) a[p] = As[st][k][wm * (T / 2) + 16 * p + (lane & 15)];
 #pragma unroll
-                for (int c = 0; c < BP; ++c) b[c] = Bs[k][wn * (T / 2) + 16 * c + (lane & 15)];
+                for (int c = 0; c < BP; ++c) b[c] = Bs[st][k][wn * (T / 2) + 16 * c + (lane & 15)];
 #pragma unroll
                 for (int p = 0; p < BP; ++p)
 #pragma unroll
                     for (int c = 0; c < BP; ++c)
                         acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p], b[c], acc[p][c], 0, 0, 0);
             }
+            // the other stage was last read in the previous chunk, before the
+            // barrier that ended it
+            if (more) stash(st ^ 1);
             __syncthreads();
+            st ^= 1;
         }
 #pragma unroll
         for (int p = 0; p < BP; ++p)

@@ -346,14 +346,29 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     }
 
     // Epilogue: acc[p][c] element q of lane l is row 16p + 4((l/4)%4) + l/16,
-    // column 16c + 4q + l%4 of the wave tile.
+    // column 16c + 4q + l%4 of the wave tile.  For doAdd / doSub the 16 old
+    // values of a row group (and of their mirrors) are read together before
+    // any store: one memory round trip per row group instead of one per
+    // element (the compiler cannot reorder the reads past the stores).
     const int erow = 4 * ((lane >> 2) & 3) + lg;
     const int ecol = lane & 3;
+    const bool mirror = upper && tm != tn;  // the mirrored element of a symmetric product (M == N)
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const int64_t row = m0 + wm * 64 + 16 * p + erow;
         if (row >= M) continue;
         double* crow = C + row * ldc;
+        double old[4][4], mold[4][4];
+        if (op != MRM_DO_NONE) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int64_t col = n0 + wn * 64 + 16 * c + 4 * q + ecol;
+                    old[c][q] = col < N ? crow[col] : 0.0;
+                    mold[c][q] = (mirror && col < N) ? C[col * ldc + row] : 0.0;
+                }
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
 #pragma unroll
@@ -361,16 +376,10 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
                 const int64_t col = n0 + wn * 64 + 16 * c + 4 * q + ecol;
                 if (col < N) {
                     const double x = acc[p][c][q];
-                    if (op == MRM_DO_NONE) crow[col] = x;
-                    else if (op == MRM_DO_ADD) crow[col] = crow[col] + x;
-                    else crow[col] = crow[col] - x;
-                    if (upper && tm != tn) {
-                        // the mirrored element of a symmetric product (M == N)
-                        double* cm = C + col * ldc + row;
-                        if (op == MRM_DO_NONE) *cm = x;
-                        else if (op == MRM_DO_ADD) *cm = *cm + x;
-                        else *cm = *cm - x;
-                    }
+                    crow[col] = op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? old[c][q] + x : old[c][q] - x);
+                    if (mirror)
+                        C[col * ldc + row] =
+                            op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? mold[c][q] + x : mold[c][q] - x);
                 }
             }
         }

@@ -370,10 +370,30 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
     uint64_t timeout = 2ull * 100000000ull;  // 2 s of wall clock (100 MHz)
     void* args[] = {&P, &lda, &m, &w, &piv_base, &indx, &xch, &flag_base, &ws.status, &timeout};
     hipError_t e;
-    const void* fn = rpt == 1   ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 1>)
-                     : rpt == 2 ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 2>)
-                                : reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 4>);
-    e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(PANEL_THREADS), args, 0, s);
+    const int ri = rpt == 1 ? 0 : rpt == 2 ? 1 : 2;
+    const void* fn = ri == 0   ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 1>)
+                     : ri == 1 ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 2>)
+                               : reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 4>);
+    // The workgroups wait for each other, so all G must be resident at once.
+    // A plain launch keeps them so whenever the device can hold G of them:
+    // the stream's earlier kernels have finished, and work on other streams
+    // does not depend on this kernel, so a delayed workgroup still starts
+    // (the deadline turns anything else into MRM_LU_TIMEOUT, never a hang).
+    // It saves the ~20 us of queue time a cooperative launch costs per panel;
+    // larger grids take the cooperative launch.  (Callers hold the host mutex.)
+    static int capacity[3] = {-1, -1, -1};
+    if (capacity[ri] < 0) {
+        int per_cu = 0, cus = 0, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PANEL_THREADS, 0) != hipSuccess)
+            per_cu = cus = 0;
+        capacity[ri] = per_cu * cus;
+    }
+    if (G <= capacity[ri])
+        e = hipLaunchKernel(fn, dim3(G), dim3(PANEL_THREADS), args, 0, s);
+    else
+        e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(PANEL_THREADS), args, 0, s);
     if (e != hipSuccess) {
         mrm_set_error(MRM_HIP_ERROR, "lu panel launch (G=%d): %s", G, hipGetErrorString(e));
         return MRM_HIP_ERROR;

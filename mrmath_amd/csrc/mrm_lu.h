@@ -14,7 +14,7 @@ inline int lu_outer_block(int mode) { return mode == MRM_SOLVE_FP64 ? LU_NB : LU
 constexpr int LU_W = 32;               // panel width of the cooperative panel kernel
 constexpr int LU_TRSM_R = 64;          // rows per diagonal block of the blocked triangular solves
 constexpr int LU_MAX_PANEL_WG = 128;   // panel workgroups (all resident; candidates fit in LDS)
-constexpr int LU_PANEL_WG_TARGET = 16; // preferred panel workgroups before packing more rows per thread
+constexpr int LU_PANEL_WG_TARGET = 32; // preferred panel workgroups before packing more rows per thread
 
 // Device workspace of the factorisation (one per library instance; users are
 // serialised by the host mutex).

@@ -19,7 +19,10 @@
  * Errors: the reference's GEMM procedures return nothing, so the GEMM entries
  * record failures in a thread-local status read with mrm_last_error() /
  * mrm_last_error_message().  LU / solve entries return the status directly.
- * Every entry is safe to call from several host threads at once.
+ * Every entry is safe to call from several host threads at once; the
+ * host-pointer GEMM entries of different threads run concurrently (each
+ * call takes a stream and staging buffers of its own from a pool), the LU /
+ * solve entries one at a time.
  */
 #ifndef MRMATH_GPU_H
 #define MRMATH_GPU_H
@@ -43,6 +46,10 @@ extern "C" {
 #define MRM_NO_MEMORY 5
 #define MRM_NOT_INITIALISED 6
 #define MRM_NOT_CONVERGED 7
+/* The LU panel's workgroups wait on each other's exchange words; a wait that
+ * passed its deadline (they were not resident together) also on the
+ * cooperative relaunch the library makes after a first timeout. */
+#define MRM_TIMED_OUT 8
 
 /* TMatrixMultDestOperation (reference src/MatrixConst.pas:147). */
 #define MRM_DO_NONE 0
@@ -66,11 +73,35 @@ extern "C" {
  * implicitly by the first entry; replaces nothing in the reference (its
  * kernels are bound at unit initialisation, src/MatrixASMStubSwitch.pas:2080-2084). */
 MRM_API int32_t mrm_init(void);
+/* Waits for the library's work and releases its streams and device buffers
+ * (also run at process exit).  No other thread may be inside the library;
+ * the next entry initialises it again.  Batched plans stay valid until
+ * mrm_batched_destroy. */
 MRM_API void mrm_shutdown(void);
 MRM_API int32_t mrm_last_error(void);
 MRM_API const char* mrm_last_error_message(void);
-/* Library version, e.g. 0x000100 = 0.1.0. */
+/* Library version, e.g. 0x000200 = 0.2.0. */
 MRM_API int32_t mrm_version(void);
+
+/* Counters of the host-pointer GEMM entries since the last reset: calls,
+ * bytes copied host -> device and device -> host, device products run.  No
+ * reference counterpart (diagnostics). */
+typedef struct mrm_stats {
+    int64_t host_calls;
+    int64_t h2d_bytes;
+    int64_t d2h_bytes;
+    int64_t device_gemms;
+} mrm_stats;
+MRM_API void mrm_get_stats(mrm_stats* out);
+MRM_API void mrm_reset_stats(void);
+
+/* Test hook, no reference counterpart: sets the deadline of the LU panel's
+ * waits on a plain (non-cooperative) launch in 100 MHz wall-clock ticks;
+ * 0 makes every such wait give up at once, so the factorisation reports a
+ * timeout and is relaunched cooperatively from the original matrix (whose
+ * waits keep the default 2 s); ticks < 0 restores the default.  Returns the
+ * number of such relaunches so far in the process. */
+MRM_API int64_t mrm_lu_panel_test_deadline(int64_t ticks);
 
 /* ---- host-pointer GEMM entries (stub-switch surface) --------------------- */
 
@@ -111,6 +142,25 @@ MRM_API void mrm_blocked_mult_t2(double* dest, int64_t destLineWidth, const doub
  * GEMM entry above; no reference counterpart (device-resident callers). */
 MRM_API int32_t mrm_dev_dgemm(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                       const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, void* stream);
+
+/* ---- threaded callers: one device GEMM instead of up to 64 tile calls ---- */
+
+/* ThrMatrixMult (reference src/ThreadedMatrixOperations.pas:729-735, the body
+ * of TThreadedMatrix.Mult, src/ThreadedMatrix.pas:347-357): dest (op)=
+ * mt1 * mt2 with width1 = height2.  The reference cuts the product into up to
+ * 64 tiles for its thread pool (ThrMatrixMultEx2, :612-718); here the whole
+ * product is one staging of each operand and one GEMM launch. */
+MRM_API void mrm_thr_mult(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
+                          int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2,
+                          int32_t op);
+/* ThrMatrixMultT1 (src/ThreadedMatrixOperations.pas:34): dest (op)= mt1^T * mt2. */
+MRM_API void mrm_thr_mult_t1(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2,
+                             int64_t width1, int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1,
+                             int64_t LineWidth2, int32_t op);
+/* ThrMatrixMultT2 (src/ThreadedMatrixOperations.pas:40): dest (op)= mt1 * mt2^T. */
+MRM_API void mrm_thr_mult_t2(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2,
+                             int64_t width1, int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1,
+                             int64_t LineWidth2, int32_t op);
 
 /* ---- FP64-accurate GEMM from INT8 slices (Ozaki scheme) ------------------ */
 
@@ -215,6 +265,28 @@ MRM_API int32_t mrm_dev_lineq_solve_ir(int32_t mode, const double* A, int64_t ld
 MRM_API int32_t mrm_lineq_solve_ex(const double* A, int64_t LineWidthA, int64_t width, const double* B,
                                    int64_t LineWidthB, double* X, int64_t LineWidthX, int64_t width2,
                                    int64_t NumRefinments, int32_t mode);
+
+/* ---- covariance (the PCA caller of the multiply) -------------------------- */
+
+/* Covariance (reference src/Statistics.pas:599-639): every row of A (height x
+ * width) is an observation, every column a variable; dest (width x width) =
+ * Ac^T Ac / m with Ac = A minus its column means and m = height - 1 when
+ * unbiased (height otherwise).  Computed without a transposed copy: the
+ * centred copy stays in device memory and Ac^T Ac takes the symmetric
+ * split-K product.  A single row or column gives its variance in dest[0]
+ * (MatrixVar, as the reference does).  Returns the status. */
+MRM_API int32_t mrm_covariance(double* dest, int64_t destLineWidth, const double* A, int64_t LineWidthA,
+                               int64_t width, int64_t height, int32_t unbiased);
+
+/* Device bytes of workspace mrm_dev_covariance needs. */
+MRM_API int64_t mrm_covariance_workspace_bytes(int64_t width, int64_t height);
+
+/* Covariance of the device matrix X (height x width, stride ldx bytes) into
+ * C (width x width, stride ldc); `work` holds the centred copy and the means
+ * (mrm_covariance_workspace_bytes).  X is left unchanged.  Both dimensions
+ * must be at least 2.  Asynchronous on `stream`. */
+MRM_API int32_t mrm_dev_covariance(const double* X, int64_t ldx, int64_t width, int64_t height, double* C,
+                                   int64_t ldc, int32_t unbiased, void* work, int64_t work_bytes, void* stream);
 
 /* ---- batched small problems (persistent launches replayed from a hipGraph) */
 

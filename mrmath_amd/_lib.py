@@ -10,7 +10,8 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libmrmath_gpu.so"
 
-MRM_OK, MRM_SINGULAR, MRM_BAD_DIMS, MRM_BAD_ARG, MRM_HIP_ERROR, MRM_NO_MEMORY, MRM_NOT_INITIALISED, MRM_NOT_CONVERGED = range(8)
+(MRM_OK, MRM_SINGULAR, MRM_BAD_DIMS, MRM_BAD_ARG, MRM_HIP_ERROR, MRM_NO_MEMORY, MRM_NOT_INITIALISED,
+ MRM_NOT_CONVERGED, MRM_TIMED_OUT) = range(9)
 DO_NONE, DO_ADD, DO_SUB = 0, 1, 2
 NN, TN, NT = 0, 1, 2
 SOLVE_FP64, SOLVE_FP32_IR, SOLVE_BF16_IR = 0, 1, 2
@@ -20,8 +21,25 @@ _i32 = C.c_int32
 _pd = C.c_void_p  # double*: passed as raw addresses
 _pv = C.c_void_p
 
+class Stats(C.Structure):
+    """mrm_stats: counters of the host-pointer GEMM entries."""
+    _fields_ = [("host_calls", C.c_int64), ("h2d_bytes", C.c_int64), ("d2h_bytes", C.c_int64),
+                ("device_gemms", C.c_int64)]
+
+
+_thr = [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i32]
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "mrm_get_stats": (None, [C.POINTER(Stats)]),
+    "mrm_reset_stats": (None, []),
+    "mrm_lu_panel_test_deadline": (_i64, [_i64]),
+    "mrm_thr_mult": (None, _thr),
+    "mrm_thr_mult_t1": (None, _thr),
+    "mrm_thr_mult_t2": (None, _thr),
+    "mrm_covariance": (_i32, [_pd, _i64, _pd, _i64, _i64, _i64, _i32]),
+    "mrm_covariance_workspace_bytes": (_i64, [_i64, _i64]),
+    "mrm_dev_covariance": (_i32, [_pd, _i64, _i64, _i64, _pd, _i64, _i32, _pv, _i64, _pv]),
     "mrm_init": (_i32, []),
     "mrm_shutdown": (None, []),
     "mrm_last_error": (_i32, []),
@@ -88,6 +106,13 @@ def load() -> C.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def stats() -> Stats:
+    """The host-entry counters (mrm_get_stats)."""
+    st = Stats()
+    load().mrm_get_stats(C.byref(st))
+    return st
 
 
 def check(code: int | None = None) -> None:

@@ -1,13 +1,15 @@
 """Builds libmrmath_gpu.so (hand-written HIP for gfx950) in-tree.
 
 Every source under ``mrmath_amd/csrc`` is compiled with hipcc for gfx950 and
-linked into ``mrmath_amd/libmrmath_gpu.so``.  Objects are rebuilt only when a
-source or header is newer; ``force=True`` rebuilds everything.
+linked into ``mrmath_amd/libmrmath_gpu.so``.  Objects are rebuilt only when
+the source or a project header it includes is newer; ``force=True`` rebuilds
+everything.
 """
 from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 from pathlib import Path
@@ -38,11 +40,30 @@ def _sources():
     return sorted(CSRC.glob("*.hip"))
 
 
-def _needs(obj: Path, src: Path, headers) -> bool:
+_INCLUDE = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.M)
+
+
+def _deps(src: Path) -> set:
+    """The project headers src includes, directly or through other headers."""
+    seen, todo = set(), [src]
+    while todo:
+        f = todo.pop()
+        for name in _INCLUDE.findall(f.read_text()):
+            for d in (f.parent, CSRC, INCLUDE):
+                h = d / name
+                if h.exists():
+                    if h not in seen:
+                        seen.add(h)
+                        todo.append(h)
+                    break
+    return seen
+
+
+def _needs(obj: Path, src: Path) -> bool:
     if not obj.exists():
         return True
     t = obj.stat().st_mtime
-    return src.stat().st_mtime > t or any(h.stat().st_mtime > t for h in headers)
+    return src.stat().st_mtime > t or any(h.stat().st_mtime > t for h in _deps(src))
 
 
 def _compile(src: Path, obj: Path):
@@ -55,13 +76,12 @@ def _compile(src: Path, obj: Path):
 
 def build(force: bool = False, verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
-    headers = list(CSRC.glob("*.h")) + list(INCLUDE.glob("*.h"))
     jobs = []
     objs = []
     for src in _sources():
         obj = OBJ / (src.stem + ".o")
         objs.append(obj)
-        if force or _needs(obj, src, headers):
+        if force or _needs(obj, src):
             jobs.append((src, obj))
     if jobs:
         workers = min(len(jobs), max(1, min(8, os.cpu_count() or 1)))

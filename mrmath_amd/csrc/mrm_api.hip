@@ -2,15 +2,20 @@
 // host-pointer entries with the reference's stub-switch signatures, and the
 // device-pointer entries.
 //
-// Host-pointer entries stage operands through grow-only device buffers owned by
-// the library.  One mutex serialises them, which makes the entries reentrant for
-// the reference's threaded callers (ThrMatrixMultEx2 calls the blocked pointer
-// from up to 64 pool threads, src/ThreadedMatrixOperations.pas:706-717).
+// Host-pointer GEMM entries take a context (a stream and grow-only staging
+// buffers) from a pool for the duration of the call, so calls from several
+// threads run concurrently: the reference's threaded callers invoke the
+// blocked pointer from up to 64 pool threads on disjoint tiles
+// (src/ThreadedMatrixOperations.pas:706-717).  The LU / solve entries share
+// one stream and staging set behind a mutex.
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "mrm_common.h"
 #include "mrm_host.h"
@@ -23,11 +28,113 @@ thread_local char t_msg[512] = "";
 
 struct Context {
     std::mutex init_mu;
-    bool ready = false;
+    std::atomic<bool> ready{false};
+    bool atexit_registered = false;
     int device = 0;
     hipStream_t stream = nullptr;
 };
 Context g_ctx;
+
+// Staging of the mutex-serialised entries (LU / solve / inverse / determinant).
+void* g_bufs[MRM_NUM_SLOTS] = {};
+size_t g_caps[MRM_NUM_SLOTS] = {};
+
+// Per-call context of a host-pointer GEMM entry.
+struct GemmCtx {
+    hipStream_t stream = nullptr;
+    void* bufs[4] = {};
+    size_t caps[4] = {};
+};
+std::mutex g_pool_mu;
+std::vector<GemmCtx*> g_pool;   // idle contexts
+int g_pool_live = 0;            // contexts handed out
+
+std::atomic<int64_t> g_calls{0}, g_h2d{0}, g_d2h{0}, g_gemms{0};
+
+int grow_buffer(void*& buf, size_t& cap, size_t bytes, hipStream_t s) {
+    if (bytes == 0) bytes = 256;
+    if (cap >= bytes) return MRM_OK;
+    if (buf) {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(buf);
+        buf = nullptr;
+        cap = 0;
+    }
+    const size_t c = (bytes + (1u << 20) - 1) & ~size_t((1u << 20) - 1);
+    hipError_t e = hipMalloc(&buf, c);
+    if (e != hipSuccess) {
+        mrm_set_error(MRM_NO_MEMORY, "hipMalloc(%zu) failed: %s", c, hipGetErrorString(e));
+        return MRM_NO_MEMORY;
+    }
+    cap = c;
+    return MRM_OK;
+}
+
+void free_ctx(GemmCtx* c) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 4; ++i)
+        if (c->bufs[i]) (void)hipFree(c->bufs[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int acquire_ctx(GemmCtx** out) {
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (!g_pool.empty()) {
+            *out = g_pool.back();
+            g_pool.pop_back();
+            ++g_pool_live;
+            return MRM_OK;
+        }
+    }
+    GemmCtx* c = new GemmCtx();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        mrm_set_error(MRM_HIP_ERROR, "hipStreamCreate failed");
+        return MRM_HIP_ERROR;
+    }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    ++g_pool_live;
+    *out = c;
+    return MRM_OK;
+}
+
+void release_ctx(GemmCtx* c) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(c);
+    --g_pool_live;
+}
+
+// Releases every library-owned HIP object (see mrm_shutdown).
+void release_all() {
+    std::lock_guard<std::mutex> lk(g_ctx.init_mu);
+    if (!g_ctx.ready.load()) return;
+    (void)hipStreamSynchronize(g_ctx.stream);
+    {
+        std::lock_guard<std::mutex> pk(g_pool_mu);
+        for (GemmCtx* c : g_pool) free_ctx(c);
+        g_pool.clear();
+    }
+    {
+        std::lock_guard<std::mutex> hk(mrm_host_mutex());
+        for (int i = 0; i < MRM_NUM_SLOTS; ++i) {
+            if (g_bufs[i]) (void)hipFree(g_bufs[i]);
+            g_bufs[i] = nullptr;
+            g_caps[i] = 0;
+        }
+        mrm::lu_release_workspace();
+        mrm::ir_release_workspace();
+    }
+    (void)hipStreamDestroy(g_ctx.stream);
+    g_ctx.stream = nullptr;
+    // give the stream-ordered allocations of the split products back
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, g_ctx.device) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+    g_ctx.ready.store(false);
+}
+
+void release_at_exit() { release_all(); }
 
 }  // namespace
 
@@ -44,44 +151,25 @@ static void clear_error() {
     t_msg[0] = 0;
 }
 
-// ---------------------------------------------------------------------------
-// Staging buffers for host-pointer entries.
-// ---------------------------------------------------------------------------
 std::mutex& mrm_host_mutex() {
     static std::mutex mu;
     return mu;
 }
 
 int mrm_staging(int slot, size_t bytes, void** out) {
-    static void* bufs[MRM_NUM_SLOTS] = {};
-    static size_t caps[MRM_NUM_SLOTS] = {};
     if (slot < 0 || slot >= MRM_NUM_SLOTS) return MRM_BAD_ARG;
-    if (bytes == 0) bytes = 256;
-    if (caps[slot] < bytes) {
-        if (bufs[slot]) {
-            (void)hipStreamSynchronize(g_ctx.stream);
-            (void)hipFree(bufs[slot]);
-            bufs[slot] = nullptr;
-            caps[slot] = 0;
-        }
-        size_t cap = (bytes + (1u << 20) - 1) & ~size_t((1u << 20) - 1);
-        hipError_t e = hipMalloc(&bufs[slot], cap);
-        if (e != hipSuccess) {
-            mrm_set_error(MRM_NO_MEMORY, "hipMalloc(%zu) failed: %s", cap, hipGetErrorString(e));
-            return MRM_NO_MEMORY;
-        }
-        caps[slot] = cap;
-    }
-    *out = bufs[slot];
+    int rc = grow_buffer(g_bufs[slot], g_caps[slot], bytes, g_ctx.stream);
+    if (rc) return rc;
+    *out = g_bufs[slot];
     return MRM_OK;
 }
 
 hipStream_t mrm_stream() { return g_ctx.stream; }
 
 int mrm_ensure_init() {
-    if (g_ctx.ready) return MRM_OK;
+    if (g_ctx.ready.load()) return MRM_OK;
     std::lock_guard<std::mutex> lk(g_ctx.init_mu);
-    if (g_ctx.ready) return MRM_OK;
+    if (g_ctx.ready.load()) return MRM_OK;
     int dev = 0;
     MRM_HIP_TRY(hipGetDevice(&dev));
     MRM_HIP_TRY(hipSetDevice(dev));
@@ -89,7 +177,13 @@ int mrm_ensure_init() {
     MRM_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     g_ctx.device = dev;
     g_ctx.stream = s;
-    g_ctx.ready = true;
+    if (!g_ctx.atexit_registered) {
+        // registered after the HIP runtime initialised, so it runs before the
+        // runtime's own teardown (and a profiler's finalisation)
+        std::atexit(release_at_exit);
+        g_ctx.atexit_registered = true;
+    }
+    g_ctx.ready.store(true);
     return MRM_OK;
 }
 
@@ -103,11 +197,48 @@ hipStream_t mrm_resolve_stream(void* s) { return reinterpret_cast<hipStream_t>(s
 static int copy_in(void* dst, const void* src, int64_t lw, int64_t w, int64_t h, hipStream_t s) {
     MRM_HIP_TRY(hipMemcpy2DAsync(dst, size_t(w) * 8, src, size_t(lw), size_t(w) * 8, size_t(h),
                                  hipMemcpyHostToDevice, s));
+    g_h2d.fetch_add(int64_t(w) * h * 8);
     return MRM_OK;
 }
 
 // dims: A is (rowsA x colsA) with LW1, B is (rowsB x colsB) with LW2, C is M x N.
-// slices == 0: native FP64 GEMM; 1..8: INT8-slice emulation (workspace in slot 3)
+// slices == 0: native FP64 GEMM; 1..8: INT8-slice emulation (workspace buffer 3)
+static int host_gemm_on(GemmCtx& cx, int trans, double* dest, int64_t destLW, const double* mt1, const double* mt2,
+                        int64_t w1, int64_t h1, int64_t w2, int64_t h2, int64_t LW1, int64_t LW2, int64_t M,
+                        int64_t N, int64_t K, int op, int slices) {
+    hipStream_t s = cx.stream;
+    int rc;
+    // one operand used twice (X^T X, X X^T) is staged once, which also lets
+    // the device product take its symmetric path
+    const bool same = mt1 == mt2 && LW1 == LW2 && w1 == w2 && h1 == h2;
+    if ((rc = grow_buffer(cx.bufs[0], cx.caps[0], size_t(w1 * h1) * 8, s))) return rc;
+    if (!same && (rc = grow_buffer(cx.bufs[1], cx.caps[1], size_t(w2 * h2) * 8, s))) return rc;
+    if ((rc = grow_buffer(cx.bufs[2], cx.caps[2], size_t(M * N) * 8, s))) return rc;
+    void* dA = cx.bufs[0];
+    void* dB = same ? dA : cx.bufs[1];
+    void* dC = cx.bufs[2];
+    if ((rc = copy_in(dA, mt1, LW1, w1, h1, s))) return rc;
+    if (!same && (rc = copy_in(dB, mt2, LW2, w2, h2, s))) return rc;
+    if (op != MRM_DO_NONE && (rc = copy_in(dC, dest, destLW, N, M, s))) return rc;
+    if (slices == 0) {
+        rc = mrm_launch_dgemm(trans, M, N, K, static_cast<const double*>(dA), w1 * 8, static_cast<const double*>(dB),
+                              w2 * 8, static_cast<double*>(dC), N * 8, op, s);
+    } else {
+        const size_t wb = mrm_ozaki_workspace(M, N, K, slices);
+        if ((rc = grow_buffer(cx.bufs[3], cx.caps[3], wb, s))) return rc;
+        rc = mrm_launch_dgemm_ozaki(trans, M, N, K, static_cast<const double*>(dA), w1 * 8,
+                                    static_cast<const double*>(dB), w2 * 8, static_cast<double*>(dC), N * 8, op, slices,
+                                    cx.bufs[3], wb, s);
+    }
+    if (rc) return rc;
+    g_gemms.fetch_add(1);
+    MRM_HIP_TRY(hipMemcpy2DAsync(dest, size_t(destLW), dC, size_t(N) * 8, size_t(N) * 8, size_t(M),
+                                 hipMemcpyDeviceToHost, s));
+    g_d2h.fetch_add(M * N * 8);
+    MRM_HIP_TRY(hipStreamSynchronize(s));
+    return MRM_OK;
+}
+
 static int host_gemm(int trans, double* dest, int64_t destLW, const double* mt1, const double* mt2, int64_t w1,
                      int64_t h1, int64_t w2, int64_t h2, int64_t LW1, int64_t LW2, int op, int slices = 0) {
     clear_error();
@@ -138,36 +269,13 @@ static int host_gemm(int trans, double* dest, int64_t destLW, const double* mt1,
     }
     int rc = mrm_ensure_init();
     if (rc) return rc;
-
-    std::lock_guard<std::mutex> lk(mrm_host_mutex());
-    hipStream_t s = mrm_stream();
-    void *dA, *dB, *dC;
-    // one operand used twice (X^T X, X X^T) is staged once, which also lets
-    // the device product take its symmetric path
-    const bool same = mt1 == mt2 && LW1 == LW2 && w1 == w2 && h1 == h2;
-    if ((rc = mrm_staging(0, size_t(w1 * h1) * 8, &dA))) return rc;
-    if (!same && (rc = mrm_staging(1, size_t(w2 * h2) * 8, &dB))) return rc;
-    if ((rc = mrm_staging(2, size_t(M * N) * 8, &dC))) return rc;
-    if ((rc = copy_in(dA, mt1, LW1, w1, h1, s))) return rc;
-    if (same) dB = dA;
-    else if ((rc = copy_in(dB, mt2, LW2, w2, h2, s))) return rc;
-    if (op != MRM_DO_NONE && (rc = copy_in(dC, dest, destLW, N, M, s))) return rc;
-    if (slices == 0) {
-        rc = mrm_launch_dgemm(trans, M, N, K, static_cast<const double*>(dA), w1 * 8, static_cast<const double*>(dB),
-                              w2 * 8, static_cast<double*>(dC), N * 8, op, s);
-    } else {
-        const size_t wb = mrm_ozaki_workspace(M, N, K, slices);
-        void* dW;
-        if ((rc = mrm_staging(3, wb, &dW))) return rc;
-        rc = mrm_launch_dgemm_ozaki(trans, M, N, K, static_cast<const double*>(dA), w1 * 8,
-                                    static_cast<const double*>(dB), w2 * 8, static_cast<double*>(dC), N * 8, op, slices,
-                                    dW, wb, s);
-    }
-    if (rc) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dest, size_t(destLW), dC, size_t(N) * 8, size_t(N) * 8, size_t(M),
-                                 hipMemcpyDeviceToHost, s));
-    MRM_HIP_TRY(hipStreamSynchronize(s));
-    return MRM_OK;
+    g_calls.fetch_add(1);
+    GemmCtx* cx;
+    if ((rc = acquire_ctx(&cx))) return rc;
+    rc = host_gemm_on(*cx, trans, dest, destLW, mt1, mt2, w1, h1, w2, h2, LW1, LW2, M, N, K, op, slices);
+    if (rc) (void)hipStreamSynchronize(cx->stream);  // nothing of this call may still run on a reused context
+    release_ctx(cx);
+    return rc;
 }
 
 extern "C" {
@@ -177,17 +285,26 @@ int32_t mrm_init(void) {
     return mrm_ensure_init();
 }
 
-void mrm_shutdown(void) {
-    std::lock_guard<std::mutex> lk(g_ctx.init_mu);
-    if (!g_ctx.ready) return;
-    (void)hipStreamSynchronize(g_ctx.stream);
-    // Staging buffers and the stream live until process exit: other threads may
-    // still hold results in flight, and the reference never tears down its tables.
-}
+void mrm_shutdown(void) { release_all(); }
 
 int32_t mrm_last_error(void) { return t_err; }
 const char* mrm_last_error_message(void) { return t_msg; }
-int32_t mrm_version(void) { return 0x000100; }
+int32_t mrm_version(void) { return 0x000200; }
+
+void mrm_get_stats(mrm_stats* out) {
+    if (!out) return;
+    out->host_calls = g_calls.load();
+    out->h2d_bytes = g_h2d.load();
+    out->d2h_bytes = g_d2h.load();
+    out->device_gemms = g_gemms.load();
+}
+
+void mrm_reset_stats(void) {
+    g_calls.store(0);
+    g_h2d.store(0);
+    g_d2h.store(0);
+    g_gemms.store(0);
+}
 
 void mrm_mult(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
               int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2) {
@@ -222,6 +339,25 @@ void mrm_blocked_mult_t2(double* dest, int64_t destLineWidth, const double* mt1,
                          int64_t LineWidth2, int64_t blockSize, int32_t op, double* mem) {
     (void)blockSize;
     (void)mem;
+    host_gemm(MRM_NT, dest, destLineWidth, mt1, mt2, width1, height1, width2, height2, LineWidth1, LineWidth2, op);
+}
+
+// ThrMatrixMult / ThrMatrixMultT1 / ThrMatrixMultT2
+// (src/ThreadedMatrixOperations.pas:24, 34, 40): the whole product as one call.
+void mrm_thr_mult(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
+                  int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2, int32_t op) {
+    host_gemm(MRM_NN, dest, destLineWidth, mt1, mt2, width1, height1, width2, height2, LineWidth1, LineWidth2, op);
+}
+
+void mrm_thr_mult_t1(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
+                     int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2,
+                     int32_t op) {
+    host_gemm(MRM_TN, dest, destLineWidth, mt1, mt2, width1, height1, width2, height2, LineWidth1, LineWidth2, op);
+}
+
+void mrm_thr_mult_t2(double* dest, int64_t destLineWidth, const double* mt1, const double* mt2, int64_t width1,
+                     int64_t height1, int64_t width2, int64_t height2, int64_t LineWidth1, int64_t LineWidth2,
+                     int32_t op) {
     host_gemm(MRM_NT, dest, destLineWidth, mt1, mt2, width1, height1, width2, height2, LineWidth1, LineWidth2, op);
 }
 

@@ -21,6 +21,15 @@ int mrm_launch_dgemm_splitk(int trans, int64_t M, int64_t N, int64_t K, const do
                             const double* B, int64_t ldb, double* C, int64_t ldc, int op, int splits,
                             double* work, hipStream_t stream);
 
+// The same with the shape's own K split and the partial products in the
+// caller's work (work_bytes checked against what the split needs).
+int mrm_launch_dgemm_ws(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
+                        int64_t ldb, double* C, int64_t ldc, int op, void* work, size_t work_bytes,
+                        hipStream_t stream);
+// Bytes of partial products mrm_launch_dgemm_ws needs for this shape (0: no
+// split); same_operand: A and B are one matrix with one stride (A^T A, A A^T).
+size_t mrm_dgemm_work_bytes(int trans, int64_t M, int64_t N, int64_t K, bool same_operand);
+
 void mrm_set_error(int code, const char* fmt, ...);
 
 #define MRM_HIP_TRY(expr)                                                                  \

@@ -436,12 +436,42 @@ __global__ void zero_matrix_kernel(double* C, int64_t ldc, int64_t M, int64_t N)
 
 namespace {
 
+// The split of a product: K chunks (1 = no split) and their chunk length.
+struct Split {
+    int used = 1;
+    int64_t kchunk = 0;
+    int upper = 0;
+};
+
+Split plan_split(int trans, int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B, int64_t ldb,
+                 int want_splits) {
+    Split sp;
+    const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+    // A^T A and A A^T are symmetric: compute the tiles on and above the
+    // diagonal and mirror them (the mirrored sums are the same products in the
+    // same order, so the result is the full product's bit for bit)
+    sp.upper = (trans != MRM_NN && A == B && lda == ldb && M == N) ? 1 : 0;
+    const int64_t ntiles = sp.upper ? tm * (tm + 1) / 2 : tm * tn;
+    int splits = splitk_count(ntiles, K);
+    if (want_splits > 0) {
+        const int64_t most = (K + BK - 1) / BK;
+        splits = int(want_splits < most ? want_splits : most);
+    }
+    if (splits > 1) {
+        sp.kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
+        sp.used = int((K + sp.kchunk - 1) / sp.kchunk);
+    }
+    return sp;
+}
+
 // C (M x N) = / += / -= op(A) * op(B), K the inner dimension.  Strides in bytes.
 // want_splits > 0 asks for that many K chunks (at least BK each), 0 lets the
-// shape decide; work (optional) holds want_splits * M * N doubles of partial
-// products, otherwise they come from a stream-ordered allocation.
+// shape decide; work (optional) holds the partial products (used * M * N
+// doubles; when work_bytes > 0 that is checked against it), otherwise they
+// come from a stream-ordered allocation.
 int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
-                 int64_t ldb, double* C, int64_t ldc, int op, int want_splits, double* work, hipStream_t stream) {
+                 int64_t ldb, double* C, int64_t ldc, int op, int want_splits, double* work, size_t work_bytes,
+                 hipStream_t stream) {
     if (M < 0 || N < 0 || K < 0 || op < 0 || op > 2 || trans < 0 || trans > 2) {
         mrm_set_error(MRM_BAD_ARG, "dgemm: bad argument (M=%lld N=%lld K=%lld op=%d trans=%d)", (long long)M,
                       (long long)N, (long long)K, op, trans);
@@ -470,22 +500,20 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
     }
     const bool vec = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 &&
                      ((lda | ldb) & 15) == 0;
-    // A^T A and A A^T are symmetric: compute the tiles on and above the
-    // diagonal and mirror them (the mirrored sums are the same products in the
-    // same order, so the result is the full product's bit for bit)
-    const int upper = (trans != MRM_NN && A == B && lda == ldb && M == N) ? 1 : 0;
-    const int64_t ntiles = upper ? tm * (tm + 1) / 2 : tm * tn;
-    int splits = splitk_count(ntiles, K);
-    if (want_splits > 0) {
-        const int64_t most = (K + BK - 1) / BK;
-        splits = int(want_splits < most ? want_splits : most);
-    }
-    if (splits > 1) {
-        const int64_t kchunk = ((K + splits - 1) / splits + BK - 1) / BK * BK;
-        const int used = int((K + kchunk - 1) / kchunk);
+    const Split sp = plan_split(trans, M, N, K, A, lda, B, ldb, want_splits);
+    const int upper = sp.upper;
+    if (sp.used > 1) {
+        const int64_t kchunk = sp.kchunk;
+        const int used = sp.used;
         const int64_t cstride = M * N;
+        const size_t need = size_t(used) * size_t(cstride) * 8;
+        if (work && work_bytes > 0 && need > work_bytes) {
+            mrm_set_error(MRM_BAD_DIMS, "dgemm: the split product needs %zu bytes of workspace, %zu given", need,
+                          work_bytes);
+            return MRM_BAD_DIMS;
+        }
         void* W = work;
-        if (!W) MRM_HIP_TRY(hipMallocAsync(&W, size_t(used) * size_t(cstride) * 8, stream));
+        if (!W) MRM_HIP_TRY(hipMallocAsync(&W, need, stream));
         double* w = static_cast<double*>(W);
         switch (trans) {
             case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, w, N, MRM_DO_NONE, int(tm), int(tn), upper, stream, used, kchunk, cstride); break;
@@ -511,11 +539,32 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
 
 int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                      const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream) {
-    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, nullptr, stream);
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, nullptr, 0, stream);
 }
 
 int mrm_launch_dgemm_splitk(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                             const double* B, int64_t ldb, double* C, int64_t ldc, int op, int splits,
                             double* work, hipStream_t stream) {
-    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, splits < 1 ? 1 : splits, work, stream);
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, splits < 1 ? 1 : splits, work, 0, stream);
+}
+
+int mrm_launch_dgemm_ws(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
+                        int64_t ldb, double* C, int64_t ldc, int op, void* work, size_t work_bytes,
+                        hipStream_t stream) {
+    if (!work || work_bytes == 0) {
+        mrm_set_error(MRM_BAD_ARG, "dgemm: no workspace given");
+        return MRM_BAD_ARG;
+    }
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, static_cast<double*>(work), work_bytes,
+                        stream);
+}
+
+size_t mrm_dgemm_work_bytes(int trans, int64_t M, int64_t N, int64_t K, bool same_operand) {
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
+    // the operand pointers only matter for the symmetric test
+    const double one = 0.0, two = 0.0;
+    const void* A = &one;
+    const void* B = same_operand ? static_cast<const void*>(&one) : static_cast<const void*>(&two);
+    const Split sp = plan_split(trans, M, N, K, A, 8, B, 8, 0);
+    return sp.used > 1 ? size_t(sp.used) * size_t(M) * size_t(N) * 8 : 0;
 }

@@ -14,3 +14,10 @@ hipStream_t mrm_resolve_stream(void* s);
 std::mutex& mrm_host_mutex();
 // Grow-only device buffer for `slot`; callers hold mrm_host_mutex().
 int mrm_staging(int slot, size_t bytes, void** out);
+
+namespace mrm {
+// Free the library-owned workspaces of the LU and the refinement; callers
+// hold mrm_host_mutex() and no work using them is in flight.
+void lu_release_workspace();
+void ir_release_workspace();
+}  // namespace mrm

@@ -24,20 +24,34 @@
 
 namespace {
 
-// out[c] = max over rows of |M(r, c)| as the bit pattern of a non-negative
-// double (which orders like the value), combined with atomicMax; out zeroed.
+// The norms are combined with atomicMax on the bit patterns of non-negative
+// doubles, which order like the values; a NaN is recorded as the quiet NaN
+// 0x7ff8..., which orders above +inf, so it reaches the host.
+constexpr unsigned long long NAN_BITS = 0x7ff8000000000000ull;
+
+__device__ __forceinline__ void max_bits(unsigned long long* out, double v) {
+    if (isnan(v)) atomicMax(out, NAN_BITS);
+    else if (v > 0.0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+// out[c] = max over rows of |M(r, c)| (NaN if the column holds one); out zeroed.
 __global__ void __launch_bounds__(256) colmax_kernel(const double* __restrict__ Mx, int64_t ld, int64_t rows,
                                                      int64_t cols, unsigned long long* __restrict__ out) {
     const int64_t r0 = int64_t(blockIdx.x) * 256;
     const int64_t r1 = r0 + 256 < rows ? r0 + 256 : rows;
     for (int64_t c = threadIdx.x % 64; c < cols; c += 64) {
         double m = 0.0;
-        for (int64_t r = r0 + threadIdx.x / 64; r < r1; r += 4) m = fmax(m, fabs(Mx[r * ld + c]));
-        if (m > 0.0) atomicMax(out + c, static_cast<unsigned long long>(__double_as_longlong(m)));
+        bool nan = false;
+        for (int64_t r = r0 + threadIdx.x / 64; r < r1; r += 4) {
+            const double a = fabs(Mx[r * ld + c]);
+            nan |= isnan(a);
+            m = fmax(m, a);
+        }
+        max_bits(out + c, nan ? NAN : m);
     }
 }
 
-// out = max_r sum_c |A(r, c)| (one wave per row)
+// out = max_r sum_c |A(r, c)| (one wave per row; a NaN in the sum propagates)
 __global__ void __launch_bounds__(256) norm_inf_kernel(const double* __restrict__ A, int64_t lda, int64_t n,
                                                        unsigned long long* __restrict__ out) {
     const int64_t r = int64_t(blockIdx.x) * 4 + threadIdx.x / 64;
@@ -46,7 +60,7 @@ __global__ void __launch_bounds__(256) norm_inf_kernel(const double* __restrict_
     double s = 0.0;
     for (int64_t c = lane; c < n; c += 64) s += fabs(A[r * lda + c]);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0 && s > 0.0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(s)));
+    if (lane == 0) max_bits(out, s);
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p, int n) {
@@ -101,18 +115,14 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
     if ((rc = mrm::lu_workspace(&ws))) return rc;
     if ((rc = mrm::lu_lowp_reserve(*ws, n, mode))) return rc;
     const int64_t ldr = nrhs;
-    MRM_HIP_TRY(hipMemcpy2DAsync(w.lu, size_t(n) * 8, A, size_t(lda) * 8, size_t(n) * 8, size_t(n),
-                                 hipMemcpyDeviceToDevice, s));
-    mrm::zero_i32(ws->status, 1, s);
-    if ((rc = mrm::lu_factor<double>(w.lu, n, n, w.indx, w.perm, *ws, s, mode))) return rc;
-    int status = 0;
-    MRM_HIP_TRY(hipMemcpyAsync(&status, ws->status, sizeof(int), hipMemcpyDeviceToHost, s));
-    MRM_HIP_TRY(hipStreamSynchronize(s));
-    if (status == MRM_SINGULAR) return MRM_SINGULAR;
-    if (status != 0) {
-        mrm_set_error(MRM_HIP_ERROR, "lu: panel barrier timed out");
-        return MRM_HIP_ERROR;
-    }
+    // the factorisation works on a copy of A, so a relaunch copies it again
+    auto copy_a = [&]() -> int {
+        MRM_HIP_TRY(hipMemcpy2DAsync(w.lu, size_t(n) * 8, A, size_t(lda) * 8, size_t(n) * 8, size_t(n),
+                                     hipMemcpyDeviceToDevice, s));
+        return MRM_OK;
+    };
+    if ((rc = copy_a())) return rc;
+    if ((rc = mrm::lu_factor_sync(w.lu, n, n, w.indx, w.perm, *ws, s, mode, copy_a))) return rc;
     // the factors serve every refinement step: invert their diagonal blocks once
     double* dinv = mrm::lu_solve_inverse_elems(n) ? w.dinv : nullptr;
     if (dinv && (rc = mrm::lu_solve_inverses(w.lu, n, n, dinv, s))) return rc;
@@ -150,7 +160,7 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
             const double den = an * val(size_t(1 + 2 * nrhs + c)) + val(size_t(1 + c));
             const double rv = val(size_t(1 + nrhs + c));
             const double e = den > 0.0 ? rv / den : rv;
-            if (!(e <= berr)) berr = e;  // NaN propagates
+            if (!(e <= berr)) berr = e;  // a NaN stays: neither converged nor improving
         }
         st.iterations = it;
         st.backward_error = berr;
@@ -165,6 +175,18 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
 }  // namespace
 
 namespace mrm {
+
+void ir_release_workspace() {
+    IrWork& w = ir_work();
+    (void)hipFree(w.lu);
+    (void)hipFree(w.r);
+    (void)hipFree(w.d);
+    (void)hipFree(w.indx);
+    (void)hipFree(w.perm);
+    (void)hipFree(w.norms);
+    (void)hipFree(w.dinv);
+    w = IrWork();
+}
 
 int lineq_solve_ir(int mode, const double* A, int64_t lda, int n, const double* B, int64_t ldb, double* X,
                    int64_t ldx, int nrhs, int max_iter, double tol, IrStats* stats, hipStream_t s) {

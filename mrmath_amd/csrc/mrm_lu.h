@@ -3,7 +3,8 @@
 
 #include "mrm_common.h"
 
-// Status a panel kernel reports when its grid barrier passed the deadline.
+// Status a panel kernel reports when a wait on its peer workgroups' exchange
+// words passed the deadline (the workgroups were not all resident at once).
 #define MRM_LU_TIMEOUT 100
 
 namespace mrm {
@@ -11,7 +12,7 @@ namespace mrm {
 constexpr int LU_NB = 256;             // outer block: K of the trailing-update GEMM (FP64)
 constexpr int LU_NB_LOWP = 512;        // outer block of the low-precision trailing update
 inline int lu_outer_block(int mode) { return mode == MRM_SOLVE_FP64 ? LU_NB : LU_NB_LOWP; }
-constexpr int LU_W = 32;               // panel width of the cooperative panel kernel
+constexpr int LU_W = 32;               // panel width of the panel kernel
 constexpr int LU_TRSM_R = 64;          // rows per diagonal block of the blocked triangular solves
 constexpr int LU_MAX_PANEL_WG = 128;   // panel workgroups (all resident; candidates fit in LDS)
 constexpr int LU_PANEL_WG_TARGET = 32; // preferred panel workgroups before packing more rows per thread
@@ -19,12 +20,33 @@ constexpr int LU_PANEL_WG_TARGET = 32; // preferred panel workgroups before pack
 // Device workspace of the factorisation (one per library instance; users are
 // serialised by the host mutex).
 struct LuWork {
-    unsigned* counter = nullptr;  // grid-barrier arrivals
     int* status = nullptr;        // MRM_OK / MRM_SINGULAR / MRM_LU_TIMEOUT
     void* xch = nullptr;          // pivot exchange area
     void* lowp = nullptr;         // packed operands of the low-precision trailing update
     size_t lowp_bytes = 0;
+    double* backup = nullptr;     // copy of an in-place caller's matrix for a relaunch
+    size_t backup_elems = 0;
+    // panel launch: 0 = plain launch when the device holds the whole grid,
+    // 1 = always cooperative (the relaunch after a timeout)
+    int coop = 0;
+    // deadline of the panel waits on a plain launch in 100 MHz wall-clock
+    // ticks; 0 makes every wait give up at once (a host-forced timeout)
+    uint64_t plain_timeout = 0;
 };
+
+constexpr uint64_t LU_PANEL_TIMEOUT_TICKS = 2ull * 100000000ull;  // 2 s
+
+// Factors A in place like lu_factor, then synchronises s and reads the
+// status.  A panel timeout (its workgroups were not co-resident) is retried
+// once with cooperative panel launches after restore() has put the original
+// matrix back into A.  Returns MRM_OK, MRM_SINGULAR, MRM_TIMED_OUT or an error.
+template <typename Restore>
+int lu_factor_sync(double* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode,
+                   Restore restore);
+// Host-selected deadline of plain panel launches (MRM_LU_TIMEOUT test hook)
+// and the count of cooperative relaunches so far.
+uint64_t lu_plain_panel_timeout();
+void lu_count_relaunch();
 
 size_t lu_xch_bytes();
 
@@ -46,8 +68,10 @@ inline int gemm_t<double>(int trans, int64_t M, int64_t N, int64_t K, const doub
 template <typename T>
 int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode = MRM_SOLVE_FP64);
 
-// Factors the m x w panel at P (w <= LU_W) with partial pivoting as one
-// cooperative launch; pivots go to indx[piv_base + j] as absolute rows.
+// Factors the m x w panel at P (w <= LU_W) with partial pivoting in one
+// launch whose workgroups exchange candidates through flagged words (plain
+// launch, or cooperative when ws.coop is set or the device cannot hold the
+// grid); pivots go to indx[piv_base + j] as absolute rows.
 template <typename T>
 int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm, LuWork& ws, hipStream_t s);
 
@@ -72,5 +96,32 @@ void zero_i32(int* p, int n, hipStream_t s);
 
 // Library-owned LU workspace (allocated on first use); caller holds mrm_host_mutex().
 int lu_workspace(LuWork** out);
+
+template <typename Restore>
+int lu_factor_sync(double* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode,
+                   Restore restore) {
+    for (int attempt = 0;; ++attempt) {
+        ws.coop = attempt;
+        zero_i32(ws.status, 1, s);
+        int rc = lu_factor<double>(A, lda, n, indx, perm, ws, s, mode);
+        ws.coop = 0;
+        if (rc) return rc;
+        int st = 0;
+        MRM_HIP_TRY(hipMemcpyAsync(&st, ws.status, sizeof(int), hipMemcpyDeviceToHost, s));
+        MRM_HIP_TRY(hipStreamSynchronize(s));
+        if (st == MRM_OK) return MRM_OK;
+        if (st == MRM_SINGULAR) return MRM_SINGULAR;
+        if (st != MRM_LU_TIMEOUT || attempt > 0) {
+            mrm_set_error(MRM_TIMED_OUT,
+                          "lu: the panel workgroups' candidate exchange passed its deadline%s",
+                          attempt ? " also with a cooperative launch" : "");
+            return MRM_TIMED_OUT;
+        }
+        // the panel's workgroups were not resident together: start again from
+        // the original matrix with cooperative panel launches
+        lu_count_relaunch();
+        if ((rc = restore())) return rc;
+    }
+}
 
 }  // namespace mrm

@@ -9,8 +9,9 @@
 // Factorisation: right-looking, blocked twice.  Outer blocks of LU_NB = 256
 // columns (512 for the low-precision trailing update) feed the GEMM with
 // K = 256 for the trailing update; each outer block is factored in panels of
-// LU_W = 32 columns by the cooperative panel kernel (mrm_lu_panel.hip), with
-// the reference's pivot rule.  The pivot sequence is recorded both as indx
+// LU_W = 32 columns by the panel kernel (mrm_lu_panel.hip: its workgroups agree
+// on every pivot through flagged exchange words), with the reference's pivot
+// rule.  The pivot sequence is recorded both as indx
 // (row i was exchanged with row indx[i] at step i, the reference's format)
 // and as a permutation vector perm (row i of P*A is row perm[i] of A), which
 // turns the solve's row interchanges (mrm_lusolve.hip) into one parallel
@@ -248,7 +249,8 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
         }
     }
     if (perm) {
-        const int in_lds = n <= 32768;
+        // the replay's copy in LDS up to 64 KiB (16384 rows), in global memory beyond
+        const int in_lds = n <= 16384;
         hipLaunchKernelGGL(perm_from_indx_kernel, dim3(1), dim3(256), in_lds ? size_t(n) * sizeof(int) : 0, s, indx,
                            perm, n, in_lds);
     }

@@ -4,6 +4,7 @@
 // (MatrixLUDecompInPlace, MatrixLUBackSubst, MatrixLinEQSolve,
 // src/LinAlgLU.pas:283-592); device-pointer entries run on a caller's stream.
 
+#include <atomic>
 #include <cstring>
 #include <vector>
 
@@ -15,40 +16,64 @@
 
 namespace mrm {
 
+namespace {
+LuWork g_lu_ws;
+bool g_lu_ready = false;
+// deadline of plain panel launches in 100 MHz ticks (the test hook sets it)
+std::atomic<uint64_t> g_plain_timeout{LU_PANEL_TIMEOUT_TICKS};
+std::atomic<int64_t> g_relaunches{0};
+}  // namespace
+
 int lu_workspace(LuWork** out) {
-    static LuWork ws;
-    static bool ready = false;
-    if (!ready) {
-        MRM_HIP_TRY(hipMalloc(&ws.counter, 256));
-        MRM_HIP_TRY(hipMalloc(&ws.status, 256));
-        MRM_HIP_TRY(hipMalloc(&ws.xch, lu_xch_bytes()));
-        ready = true;
+    if (!g_lu_ready) {
+        MRM_HIP_TRY(hipMalloc(&g_lu_ws.status, 256));
+        MRM_HIP_TRY(hipMalloc(&g_lu_ws.xch, lu_xch_bytes()));
+        g_lu_ready = true;
     }
-    *out = &ws;
+    g_lu_ws.plain_timeout = g_plain_timeout.load();
+    *out = &g_lu_ws;
     return MRM_OK;
 }
+
+void lu_release_workspace() {
+    if (!g_lu_ready) return;
+    (void)hipFree(g_lu_ws.status);
+    (void)hipFree(g_lu_ws.xch);
+    if (g_lu_ws.lowp) (void)hipFree(g_lu_ws.lowp);
+    if (g_lu_ws.backup) (void)hipFree(g_lu_ws.backup);
+    g_lu_ws = LuWork();
+    g_lu_ready = false;
+}
+
+uint64_t lu_plain_panel_timeout() { return g_plain_timeout.load(); }
+void lu_count_relaunch() { g_relaunches.fetch_add(1); }
 
 }  // namespace mrm
 
 namespace {
 
-// Factor on the device, then read the kernels' status (synchronises s).
-int factor_and_status(double* dA, int64_t lda, int n, int* dindx, int* dperm, hipStream_t s) {
+// Factor the staged copy dA on the device and read the status (synchronises
+// s); a panel timeout is retried after restore() has put the original matrix
+// back into dA.
+template <typename Restore>
+int factor_and_status(double* dA, int64_t lda, int n, int* dindx, int* dperm, hipStream_t s, Restore restore) {
     mrm::LuWork* ws;
     int rc = mrm::lu_workspace(&ws);
     if (rc) return rc;
-    mrm::zero_i32(ws->status, 1, s);
-    rc = mrm::lu_factor<double>(dA, lda, n, dindx, dperm, *ws, s);
+    return mrm::lu_factor_sync(dA, lda, n, dindx, dperm, *ws, s, MRM_SOLVE_FP64, restore);
+}
+
+// Host-pointer entries: the original is the caller's host matrix.
+int factor_host_copy(double* dA, int64_t n, const double* A, int64_t LineWidthA, int* dindx, int* dperm,
+                     hipStream_t s) {
+    auto upload = [&]() -> int {
+        MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
+                                     hipMemcpyHostToDevice, s));
+        return MRM_OK;
+    };
+    int rc = upload();
     if (rc) return rc;
-    int st = 0;
-    MRM_HIP_TRY(hipMemcpyAsync(&st, ws->status, sizeof(int), hipMemcpyDeviceToHost, s));
-    MRM_HIP_TRY(hipStreamSynchronize(s));
-    if (st == MRM_SINGULAR) return MRM_SINGULAR;
-    if (st != 0) {
-        mrm_set_error(MRM_HIP_ERROR, "lu: panel barrier timed out");
-        return MRM_HIP_ERROR;
-    }
-    return MRM_OK;
+    return factor_and_status(dA, n, int(n), dindx, dperm, s, upload);
 }
 
 int check_square(int64_t n, int64_t lw) {
@@ -78,6 +103,11 @@ std::vector<int> perm_from_indx(const int32_t* indx, int64_t n) {
 
 extern "C" {
 
+int64_t mrm_lu_panel_test_deadline(int64_t ticks) {
+    mrm::g_plain_timeout.store(ticks < 0 ? mrm::LU_PANEL_TIMEOUT_TICKS : uint64_t(ticks));
+    return mrm::g_relaunches.load();
+}
+
 int32_t mrm_dev_lu_factor(double* A, int64_t lda, int64_t n, int32_t* indx, int32_t* perm, void* stream) {
     int rc = mrm_ensure_init();
     if (rc) return rc;
@@ -88,7 +118,30 @@ int32_t mrm_dev_lu_factor(double* A, int64_t lda, int64_t n, int32_t* indx, int3
         return MRM_BAD_ARG;
     }
     std::lock_guard<std::mutex> lk(mrm_host_mutex());
-    return factor_and_status(A, lda / 8, int(n), indx, perm, mrm_resolve_stream(stream));
+    hipStream_t s = mrm_resolve_stream(stream);
+    mrm::LuWork* ws;
+    if ((rc = mrm::lu_workspace(&ws))) return rc;
+    // A is factored in place: keep a copy for the relaunch after a panel timeout
+    const size_t elems = size_t(n) * size_t(n);
+    if (ws->backup_elems < elems) {
+        if (ws->backup) MRM_HIP_TRY(hipFree(ws->backup));
+        ws->backup = nullptr;
+        ws->backup_elems = 0;
+        if (hipMalloc(&ws->backup, elems * 8) != hipSuccess) {
+            mrm_set_error(MRM_NO_MEMORY, "lu: cannot allocate %zu bytes for the copy of A", elems * 8);
+            return MRM_NO_MEMORY;
+        }
+        ws->backup_elems = elems;
+    }
+    double* backup = ws->backup;
+    MRM_HIP_TRY(hipMemcpy2DAsync(backup, size_t(n) * 8, A, size_t(lda), size_t(n) * 8, size_t(n),
+                                 hipMemcpyDeviceToDevice, s));
+    auto restore = [&]() -> int {
+        MRM_HIP_TRY(hipMemcpy2DAsync(A, size_t(lda), backup, size_t(n) * 8, size_t(n) * 8, size_t(n),
+                                     hipMemcpyDeviceToDevice, s));
+        return MRM_OK;
+    };
+    return factor_and_status(A, lda / 8, int(n), indx, perm, s, restore);
 }
 
 int32_t mrm_dev_lu_solve(const double* LU, int64_t lda, int64_t n, const int32_t* perm, const double* B, int64_t ldb,
@@ -121,9 +174,7 @@ int32_t mrm_lu_decomp_inplace(double* A, int64_t LineWidthA, int64_t width, int3
     void *dA, *dI;
     if ((rc = mrm_staging(0, size_t(n * n) * 8, &dA))) return rc;
     if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
-    rc = factor_and_status(static_cast<double*>(dA), n, int(n), static_cast<int*>(dI), nullptr, s);
+    rc = factor_host_copy(static_cast<double*>(dA), n, A, LineWidthA, static_cast<int*>(dI), nullptr, s);
     if (rc != MRM_OK && rc != MRM_SINGULAR) return rc;
     // the reference leaves A partially factored on leSingular too
     MRM_HIP_TRY(hipMemcpy2DAsync(A, size_t(LineWidthA), dA, size_t(n) * 8, size_t(n) * 8, size_t(n),
@@ -161,9 +212,8 @@ int32_t mrm_inverse_inplace(double* A, int64_t LineWidthA, int64_t width) {
     if ((rc = mrm_staging(2, size_t(n * n) * 8, &dX))) return rc;
     if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
     if ((rc = mrm_staging(6, size_t(n) * 4, &dP))) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dLU, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
-    rc = factor_and_status(static_cast<double*>(dLU), n, int(n), static_cast<int*>(dI), static_cast<int*>(dP), s);
+    rc = factor_host_copy(static_cast<double*>(dLU), n, A, LineWidthA, static_cast<int*>(dI),
+                          static_cast<int*>(dP), s);
     if (rc) return rc;  // leSingular leaves A unchanged, as the reference does
     hipLaunchKernelGGL(identity_kernel, dim3(unsigned((n * n + 255) / 256)), dim3(256), 0, s,
                        static_cast<double*>(dE), n);
@@ -195,9 +245,7 @@ int32_t mrm_determinant(const double* A, int64_t LineWidthA, int64_t width, doub
     void *dLU, *dI;
     if ((rc = mrm_staging(0, size_t(n * n) * 8, &dLU))) return rc;
     if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dLU, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
-    rc = factor_and_status(static_cast<double*>(dLU), n, int(n), static_cast<int*>(dI), nullptr, s);
+    rc = factor_host_copy(static_cast<double*>(dLU), n, A, LineWidthA, static_cast<int*>(dI), nullptr, s);
     if (rc == MRM_SINGULAR) {
         *det = 0.0;
         return MRM_SINGULAR;
@@ -282,7 +330,11 @@ int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, cons
     if ((rc = mrm_staging(7, size_t(n * m) * 8, &dD))) return rc;
     MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
                                  hipMemcpyHostToDevice, s));
-    MRM_HIP_TRY(hipMemcpyAsync(dLU, dA, size_t(n * n) * 8, hipMemcpyDeviceToDevice, s));
+    auto copy_a = [&]() -> int {
+        MRM_HIP_TRY(hipMemcpyAsync(dLU, dA, size_t(n * n) * 8, hipMemcpyDeviceToDevice, s));
+        return MRM_OK;
+    };
+    if ((rc = copy_a())) return rc;
     MRM_HIP_TRY(hipMemcpy2DAsync(dB, size_t(m) * 8, B, size_t(LineWidthB), size_t(m) * 8, size_t(n),
                                  hipMemcpyHostToDevice, s));
     double* a = static_cast<double*>(dA);
@@ -292,7 +344,7 @@ int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, cons
     double* r = static_cast<double*>(dR);
     double* d = static_cast<double*>(dD);
     int* perm = static_cast<int*>(dP);
-    rc = factor_and_status(lu, n, int(n), static_cast<int*>(dI), perm, s);
+    rc = factor_and_status(lu, n, int(n), static_cast<int*>(dI), perm, s, copy_a);
     if (rc) return rc;
     // the diagonal-block inverses of the factors serve the solve and every
     // refinement step

@@ -1,4 +1,4 @@
-// mrm_lu_panel.hip -- the cooperative panel kernel of the LU factorisation.
+// mrm_lu_panel.hip -- the panel kernel of the LU factorisation.
 //
 // Factors an m x w (w <= LU_W) column panel in place with partial pivoting:
 // the left-looking column loop of the reference's recursive LU at its base
@@ -179,9 +179,10 @@ __device__ __forceinline__ bool panel_column(const int j, T (&v)[RPT][W], const 
     // thread watches
     const int nrec = G * R::RS;
     const int total = nrec + (own_j ? 0 : R::DU);
-    bool late = false;
+    // deadline 0: the host asked every wait to give up at once
+    bool late = deadline == 0;
     constexpr int PB = 8;  // words in flight per thread
-    for (int e0 = t; e0 < total; e0 += PB * PANEL_THREADS) {
+    for (int e0 = t; e0 < total && !late; e0 += PB * PANEL_THREADS) {
         uint64_t u[PB];
 #pragma unroll
         for (int i = 0; i < PB; ++i) {
@@ -328,7 +329,7 @@ lu_panel_kernel(T* __restrict__ P, int64_t lda, int m, int w, int piv_base, int*
 #pragma unroll
         for (int c = 0; c < W; ++c) v[r][c] = (ok && c < w) ? src[c] : T(0);
     }
-    const uint64_t deadline = wall_clock64() + timeout_ticks;
+    const uint64_t deadline = timeout_ticks ? wall_clock64() + timeout_ticks : 0;
     bool alive = true;
 
     PanelColumns<T, W, RPT, 0>::run(w, alive, v, lr, m, g, G, t, lane, wave, piv_base, indx, xch, flag_base, status,
@@ -367,7 +368,7 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
     // flags are unique within a factorisation (the exchange area is zeroed at
     // its start): column j of this panel is flagged piv_base + j + 1
     uint32_t flag_base = uint32_t(piv_base) + 1u;
-    uint64_t timeout = 2ull * 100000000ull;  // 2 s of wall clock (100 MHz)
+    uint64_t timeout = ws.coop ? LU_PANEL_TIMEOUT_TICKS : ws.plain_timeout;
     void* args[] = {&P, &lda, &m, &w, &piv_base, &indx, &xch, &flag_base, &ws.status, &timeout};
     hipError_t e;
     const int ri = rpt == 1 ? 0 : rpt == 2 ? 1 : 2;
@@ -375,22 +376,33 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
                      : ri == 1 ? reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 2>)
                                : reinterpret_cast<const void*>(&lu_panel_kernel<T, W, 4>);
     // The workgroups wait for each other, so all G must be resident at once.
-    // A plain launch keeps them so whenever the device can hold G of them:
-    // the stream's earlier kernels have finished, and work on other streams
-    // does not depend on this kernel, so a delayed workgroup still starts
-    // (the deadline turns anything else into MRM_LU_TIMEOUT, never a hang).
-    // It saves the ~20 us of queue time a cooperative launch costs per panel;
-    // larger grids take the cooperative launch.  (Callers hold the host mutex.)
-    static int capacity[3] = {-1, -1, -1};
-    if (capacity[ri] < 0) {
-        int per_cu = 0, cus = 0, dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PANEL_THREADS, 0) != hipSuccess)
-            per_cu = cus = 0;
-        capacity[ri] = per_cu * cus;
+    // A plain launch keeps them so whenever the device can hold G of them and
+    // no other work holds its CUs: the stream's earlier kernels have
+    // finished, and work on other streams does not depend on this kernel.  It
+    // saves the ~20 us of queue time a cooperative launch costs per panel.
+    // If other work did hold CUs, a wait passes its deadline (never a hang),
+    // the factorisation reports MRM_LU_TIMEOUT and lu_factor_sync relaunches
+    // it with ws.coop set: every panel then takes the cooperative launch,
+    // which guarantees co-residency.  Larger grids are always cooperative.
+    // (Callers hold the host mutex.)
+    constexpr int MAX_DEV = 64;
+    static int capacity[MAX_DEV][3];
+    static bool known[MAX_DEV][3];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV) dev = -1;
+    int cap = 0;
+    if (dev >= 0) {
+        if (!known[dev][ri]) {
+            int per_cu = 0, cus = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PANEL_THREADS, 0) != hipSuccess)
+                per_cu = cus = 0;
+            capacity[dev][ri] = per_cu * cus;
+            known[dev][ri] = true;
+        }
+        cap = capacity[dev][ri];
     }
-    if (G <= capacity[ri])
+    if (!ws.coop && G <= cap)
         e = hipLaunchKernel(fn, dim3(G), dim3(PANEL_THREADS), args, 0, s);
     else
         e = hipLaunchCooperativeKernel(fn, dim3(G), dim3(PANEL_THREADS), args, 0, s);

@@ -7,7 +7,9 @@ bytes, possibly padded).  Every call goes through the C ABI of
 libmrmath_gpu.so; there is no CPU path.
 
 Differences from the reference, by design:
-  * dimension errors raise MrmError (the reference asserts, then misbehaves);
+  * dimension errors raise (the reference asserts, then misbehaves): a dest
+    without the product's shape raises ValueError before any call, operands
+    whose sizes do not fit together raise MrmError from the library;
   * blockSize / mem are accepted and ignored (the GPU tiles itself).
 """
 from __future__ import annotations
@@ -35,11 +37,32 @@ def _mat(a: np.ndarray, name: str) -> tuple[int, int, int, int]:
     return a.ctypes.data, a.shape[1], a.shape[0], a.strides[0]
 
 
-def _call_mult(fn, dest, mt1, mt2, *extra):
-    d, _, _, dlw = _mat(dest, "dest")
+def _product_shape(trans: int, w1: int, h1: int, w2: int, h2: int) -> tuple[int, int]:
+    """(height, width) of the product for the reference's dimension rules:
+    NN h1 x w2, T1 (mt1^T mt2) w1 x w2, T2 (mt1 mt2^T) h1 x h2
+    (src/MatrixASMStubSwitch.pas:744, 786; src/BlockedMult.pas:502, 622)."""
+    if trans == _lib.NN:
+        return h1, w2
+    if trans == _lib.TN:
+        return w1, w2
+    return h1, h2
+
+
+def _operands(trans, dest, mt1, mt2):
+    """Addresses and sizes of the three operands; dest must have exactly the
+    product's shape (the library writes height x width rows at dest's stride,
+    so a smaller dest would be written past its end)."""
+    d, dw, dh, dlw = _mat(dest, "dest")
     p1, w1, h1, lw1 = _mat(mt1, "mt1")
     p2, w2, h2, lw2 = _mat(mt2, "mt2")
-    fn(d, dlw, p1, p2, w1, h1, w2, h2, lw1, lw2, *extra)
+    want = _product_shape(trans, w1, h1, w2, h2)
+    if (dh, dw) != want:
+        raise ValueError(f"Dimension error: dest is {dh} x {dw}, the product is {want[0]} x {want[1]}")
+    return d, dlw, p1, p2, w1, h1, w2, h2, lw1, lw2
+
+
+def _call_mult(fn, trans, dest, mt1, mt2, *extra):
+    fn(*_operands(trans, dest, mt1, mt2), *extra)
     _lib.check()
     return dest
 
@@ -49,23 +72,23 @@ def MatrixMult(dest: np.ndarray, mt1: np.ndarray, mt2: np.ndarray) -> np.ndarray
     lib = _lib.load()
     dims = (mt1.shape[1], mt1.shape[0], mt2.shape[1], mt2.shape[0])
     if min(dims) >= BlockedMatrixMultSize:
-        return _call_mult(lib.mrm_blocked_mult, dest, mt1, mt2, BlockMatrixCacheSize, doNone, None)
-    return _call_mult(lib.mrm_mult, dest, mt1, mt2)
+        return _call_mult(lib.mrm_blocked_mult, _lib.NN, dest, mt1, mt2, BlockMatrixCacheSize, doNone, None)
+    return _call_mult(lib.mrm_mult, _lib.NN, dest, mt1, mt2)
 
 
 def MatrixMultEx(dest, mt1, mt2, blockSize: int = BlockMatrixCacheSize, op: int = doNone, mem=None):
     """dest (op)= mt1 * mt2 via the blocked pointer (src/MatrixASMStubSwitch.pas:778-781)."""
-    return _call_mult(_lib.load().mrm_blocked_mult, dest, mt1, mt2, blockSize, op, None)
+    return _call_mult(_lib.load().mrm_blocked_mult, _lib.NN, dest, mt1, mt2, blockSize, op, None)
 
 
 def MatrixMultT1(dest, mt1, mt2):
     """dest = mt1^T * mt2 (src/MatrixASMStubSwitch.pas:812-817)."""
-    return _call_mult(_lib.load().mrm_blocked_mult_t1, dest, mt1, mt2, BlockMatrixCacheSize, doNone, None)
+    return _call_mult(_lib.load().mrm_blocked_mult_t1, _lib.TN, dest, mt1, mt2, BlockMatrixCacheSize, doNone, None)
 
 
 def MatrixMultT1Ex(dest, mt1, mt2, blockSize: int = BlockMatrixCacheSize, op: int = doNone, mem=None):
     """dest (op)= mt1^T * mt2 (src/MatrixASMStubSwitch.pas:819-822)."""
-    return _call_mult(_lib.load().mrm_blocked_mult_t1, dest, mt1, mt2, blockSize, op, None)
+    return _call_mult(_lib.load().mrm_blocked_mult_t1, _lib.TN, dest, mt1, mt2, blockSize, op, None)
 
 
 def MatrixMultOzaki(dest, mt1, mt2, slices: int = 7, trans: int = _lib.NN, op: int = doNone):
@@ -75,21 +98,36 @@ def MatrixMultOzaki(dest, mt1, mt2, slices: int = 7, trans: int = _lib.NN, op: i
     <= 4 (slices+1) K 2^(-7 slices) max|row of op(mt1)| max|column of op(mt2)|
     plus FP64 rounding.  No reference counterpart (the reference multiplies in
     FP64 only)."""
-    d, _, _, dlw = _mat(dest, "dest")
-    p1, w1, h1, lw1 = _mat(mt1, "mt1")
-    p2, w2, h2, lw2 = _mat(mt2, "mt2")
-    _lib.check(_lib.load().mrm_mult_ozaki(trans, d, dlw, p1, p2, w1, h1, w2, h2, lw1, lw2, op, slices))
+    if trans not in (_lib.NN, _lib.TN, _lib.NT):
+        raise ValueError(f"trans must be NN, TN or NT, not {trans}")
+    _lib.check(_lib.load().mrm_mult_ozaki(trans, *_operands(trans, dest, mt1, mt2), op, slices))
     return dest
 
 
 def MatrixMultT2(dest, mt1, mt2):
     """dest = mt1 * mt2^T via multT2Func (src/MatrixASMStubSwitch.pas:853-858)."""
-    return _call_mult(_lib.load().mrm_mult_t2, dest, mt1, mt2)
+    return _call_mult(_lib.load().mrm_mult_t2, _lib.NT, dest, mt1, mt2)
 
 
 def MatrixMultT2Ex(dest, mt1, mt2, blockSize: int = BlockMatrixCacheSize, op: int = doNone, mem=None):
     """dest (op)= mt1 * mt2^T (src/MatrixASMStubSwitch.pas:860-863)."""
-    return _call_mult(_lib.load().mrm_blocked_mult_t2, dest, mt1, mt2, blockSize, op, None)
+    return _call_mult(_lib.load().mrm_blocked_mult_t2, _lib.NT, dest, mt1, mt2, blockSize, op, None)
+
+
+def ThrMatrixMult(dest, mt1, mt2, op: int = doNone):
+    """dest (op)= mt1 * mt2 (src/ThreadedMatrixOperations.pas:729-735): one
+    device product for the whole matrix instead of the thread pool's tiles."""
+    return _call_mult(_lib.load().mrm_thr_mult, _lib.NN, dest, mt1, mt2, op)
+
+
+def ThrMatrixMultT1(dest, mt1, mt2, op: int = doNone):
+    """dest (op)= mt1^T * mt2 (src/ThreadedMatrixOperations.pas:34)."""
+    return _call_mult(_lib.load().mrm_thr_mult_t1, _lib.TN, dest, mt1, mt2, op)
+
+
+def ThrMatrixMultT2(dest, mt1, mt2, op: int = doNone):
+    """dest (op)= mt1 * mt2^T (src/ThreadedMatrixOperations.pas:40)."""
+    return _call_mult(_lib.load().mrm_thr_mult_t2, _lib.NT, dest, mt1, mt2, op)
 
 
 def aligned_matrix(width: int, height: int) -> np.ndarray:
@@ -133,4 +171,14 @@ class TDoubleMatrix:
     def MultT2(self, Value: "TDoubleMatrix") -> "TDoubleMatrix":
         res = TDoubleMatrix(Value.Height, self.Height)
         MatrixMultT2Ex(res.data, self.data, Value.data, BlockMatrixCacheSize, doNone)
+        return res
+
+
+class TThreadedMatrix(TDoubleMatrix):
+    """Mirror of TThreadedMatrix.Mult (src/ThreadedMatrix.pas:347-357): the
+    result of the whole product through ThrMatrixMult."""
+
+    def Mult(self, Value: "TDoubleMatrix") -> "TThreadedMatrix":
+        res = TThreadedMatrix(Value.Width, self.Height)
+        ThrMatrixMult(res.data, self.data, Value.data)
         return res

@@ -45,6 +45,7 @@ _SIG = {
     "orc_determinant": (C.c_double, [_p, _i64, _i64]),
     "orc_gemm_normwise_err": (C.c_double, [_p, _i64, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64]),
     "orc_backward_error": (C.c_double, [_p, _i64, _i64, _p, _i64, _p, _i64, _i64]),
+    "orc_covariance": (None, [_p, _i64, _p, _i64, _i64, _i64, _i32]),
     "orc_fast_mult": (C.c_int, [_p, _i64, _p, _p, _i64, _i64, _i64, _i64, _i64, _i64, C.c_int, C.c_int]),
     "orc_num_cores_for_mult": (C.c_int, [_i64, _i64, _i64, _i64, C.c_int]),
 }
@@ -242,3 +243,14 @@ def backward_error(A: np.ndarray, X: np.ndarray, B: np.ndarray) -> float:
     px, nrhs, _, lwx = _m(X)
     pb, _, _, lwb = _m(B)
     return lib().orc_backward_error(pa, lwa, n, px, lwx, pb, lwb, nrhs)
+
+
+# ---- statistics -----------------------------------------------------------------
+def covariance(A: np.ndarray, unbiased: bool = True) -> np.ndarray:
+    """Covariance (src/Statistics.pas:599-639): width x width, or the 1 x 1
+    variance of a single row or column."""
+    pa, w, h, lwa = _m(A)
+    n = 1 if (w == 1 or h == 1) else w
+    D = np.zeros((n, n))
+    lib().orc_covariance(D.ctypes.data, n * 8, pa, lwa, w, h, 1 if unbiased else 0)
+    return D

@@ -540,3 +540,49 @@ double orc_backward_error(const double* A, int64_t LWA, int64_t n, const double*
     free(r);
     return worst;
 }
+
+/* Covariance (src/Statistics.pas:599-639): column means by a sequential sum
+ * over the rows divided by the height (GenericMtxMean, column-wise,
+ * src/SimpleMatrixOperations.pas:432-445); ac = A - mean (MatrixSubVec);
+ * dest = ac^T * ac as the generic product of the transposed copy
+ * (MatrixTranspose + MatrixMult, :626-632); then dest = dest * (1/m) + 0
+ * (MatrixScaleAndAdd, :633; GenericMtxScaleAndAdd,
+ * src/SimpleMatrixOperations.pas:1220-1232), m = height - 1 when unbiased.
+ * A single row or column gives MatrixVar's value in dest[0]
+ * (GenericMtxVar, src/SimpleMatrixOperations.pas:450-520). */
+void orc_covariance(double* dest, int64_t LWD, const double* A, int64_t LWA, int64_t width, int64_t height,
+                    int32_t unbiased)
+{
+    if (height == 1 || width == 1) {
+        const int64_t n = width * height;
+        double val = 0.0;
+        for (int64_t i = 0; i < n; i++) val = val + (height == 1 ? *CELEM(A, LWA, i, 0) : *CELEM(A, LWA, 0, i));
+        const double mean = val / (double)n;
+        double v = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            const double d = (height == 1 ? *CELEM(A, LWA, i, 0) : *CELEM(A, LWA, 0, i)) - mean;
+            v = v + d * d;
+        }
+        dest[0] = unbiased ? v / (double)(n - 1 > 1 ? n - 1 : 1) : v / (double)n;
+        return;
+    }
+    double* mean = (double*)malloc(sizeof(double) * width);
+    double* ac = (double*)malloc(sizeof(double) * width * height);
+    double* tmp = (double*)malloc(sizeof(double) * width * height);
+    for (int64_t x = 0; x < width; x++) {
+        double val = 0.0;
+        for (int64_t y = 0; y < height; y++) val = val + *CELEM(A, LWA, x, y);
+        mean[x] = val / (double)height;
+    }
+    for (int64_t y = 0; y < height; y++)
+        for (int64_t x = 0; x < width; x++) ac[y * width + x] = *CELEM(A, LWA, x, y) - mean[x];
+    /* tmp (width x height) = ac^T */
+    for (int64_t y = 0; y < height; y++)
+        for (int64_t x = 0; x < width; x++) tmp[x * height + y] = ac[y * width + x];
+    orc_generic_mtx_mult(dest, LWD, tmp, ac, height, width, width, height, height * 8, width * 8);
+    const int64_t m = unbiased ? height - 1 : height;
+    const double scale = 1.0 / (double)m;
+    for (int64_t y = 0; y < width; y++)
+        for (int64_t x = 0; x < width; x++) *ELEM(dest, LWD, x, y) = (*ELEM(dest, LWD, x, y) * scale) + 0.0;
+    free(mean); free(ac); free(tmp);
+}

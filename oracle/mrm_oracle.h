@@ -96,6 +96,9 @@ double orc_gemm_normwise_err(const double* C, int64_t LWC, const double* Cref, i
 int32_t orc_ozaki_gemm(double* C, int64_t LWC, const double* A, int64_t LWA, const double* B, int64_t LWB,
                        int64_t M, int64_t N, int64_t K, int32_t op, int32_t slices, int64_t kchunk);
 /* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf), residual in long double. */
+/* Covariance (src/Statistics.pas:599-639) */
+void orc_covariance(double* dest, int64_t LWD, const double* A, int64_t LWA, int64_t width, int64_t height,
+                    int32_t unbiased);
 double orc_backward_error(const double* A, int64_t LWA, int64_t n, const double* X, int64_t LWX,
                           const double* B, int64_t LWB, int64_t nrhs);
 

@@ -309,3 +309,26 @@ def test_symmetric_host_entry_stages_once():
         mult(C2, Y, Y.copy())
         assert np.array_equal(C1, C2)
         assert np.array_equal(C1, C1.T)
+
+
+def test_reference_fixture_products():
+    """TestMult / TestMultT1 / TestMultT2 (Test/TestMatrixClass.pas:456-559) on
+    the reference's fixtures MatrixData1/2 (read with the RMCLS reader): the
+    dimension checks of the reference tests, and the values, which are
+    integer-valued and therefore exact."""
+    from pathlib import Path
+    from mrmath_amd import persistence as ps
+    golden = Path(__file__).resolve().parent / "golden"
+    m1 = ps.read_matrix(golden / "MatrixData1.txt")
+    m2 = ps.read_matrix(golden / "MatrixData2.txt")
+    r1 = ss.TDoubleMatrix(m1.Width, m1.Height, m1.SubMatrix())
+    r2t = ss.TDoubleMatrix(m2.Height, m2.Width, m2.SubMatrix().T)   # fRefMatrix2.Transpose
+    mtx = r1.Mult(r2t)
+    assert mtx.Width == mtx.Height == 10
+    assert np.array_equal(mtx.data, m1.data @ m2.data.T)
+    mtx = r1.MultT1(ss.TDoubleMatrix(m1.Width, m1.Height, m1.SubMatrix()))
+    assert mtx.Width == mtx.Height == 5
+    assert np.array_equal(mtx.data, m1.data.T @ m1.data)
+    mtx = r1.MultT2(ss.TDoubleMatrix(m1.Width, m1.Height, m1.SubMatrix()))
+    assert mtx.Width == mtx.Height == 10
+    assert np.array_equal(mtx.data, m1.data @ m1.data.T)

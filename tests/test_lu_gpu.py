@@ -12,6 +12,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import _pins
 import oracle as orc
 from mrmath_amd import _lib
 from mrmath_amd import linalg as la
@@ -106,19 +107,142 @@ def test_lineq_solve_backward_error(n, nrhs):
     assert np.allclose(X, Xo, rtol=1e-8, atol=1e-10)
 
 
-def test_lineq_solve_refinement():
-    n, nrhs = 400, 3
-    rng = np.random.default_rng(5)
-    Uq, _ = np.linalg.qr(rng.standard_normal((n, n)))
-    Vq, _ = np.linalg.qr(rng.standard_normal((n, n)))
-    A = Uq @ np.diag(np.logspace(0, -6, n)) @ Vq.T
-    B = rng.standard_normal((n, nrhs))
-    X0 = np.zeros((n, nrhs))
-    X2 = np.zeros((n, nrhs))
-    assert la.MatrixLinEQSolve(A, B, X0, 0) == la.leOk
-    assert la.MatrixLinEQSolve(A, B, X2, 2) == la.leOk
-    assert orc.backward_error(A, X2, B) <= 1e-14
-    assert orc.backward_error(A, X2, B) <= orc.backward_error(A, X0, B) * 1.5 + 1e-17
+@pytest.fixture(scope="module")
+def wilkinson_system():
+    n = 50
+    A = _pins.wilkinson(n)
+    B = orc.pm1_matrix(3, n, 7)
+    Xe = np.stack([_pins.exact_solve(A, B[:, j]) for j in range(3)], axis=1)
+    return A, B, Xe
+
+
+def test_lineq_solve_refinement_pinned(wilkinson_system):
+    """NumRefinments on the GPU (src/LinAlgLU.pas:542-586) against the exact
+    rational solution of the n = 50 Wilkinson system (pivot growth 2^49):
+    unrefined the forward error is ~1e-3, one FP64 step brings it to the
+    rounding level, more steps keep it there; 0 and 1 refinements differ.  A
+    refinement that does nothing or has the wrong sign fails."""
+    A, B, Xe = wilkinson_system
+    X = [np.zeros_like(B) for _ in range(3)]
+    for k in range(3):
+        assert la.MatrixLinEQSolve(A, B, X[k], k) == la.leOk
+    errs = [_pins.forward_error(x, Xe) for x in X]
+    print("forward errors by refinement steps:", errs)
+    assert errs[0] > 1e-5
+    assert errs[1] <= 2e-15 and errs[2] <= 2e-15
+    assert np.abs(X[1] - X[0]).max() > 1e3 * np.abs(X[1] - Xe).max()
+    assert orc.backward_error(A, X[1], B) < 1e-16
+    # single right-hand side in the reference's layout (LineWidthX = 8)
+    x1 = np.zeros((50, 1))
+    assert la.MatrixLinEQSolve(A, np.ascontiguousarray(B[:, :1]), x1, 1) == la.leOk
+    assert _pins.forward_error(x1, Xe[:, :1]) <= 2e-15
+
+
+def test_refinement_entry_steps_pinned(wilkinson_system):
+    """The device refinement driver on the same system: with no step allowed
+    it reports the unrefined backward error and MRM_NOT_CONVERGED; allowed
+    steps converge in one or two, to the exact solution's rounding level."""
+    import ctypes as C
+    import torch
+    lib = _lib.load()
+    A, B, Xe = wilkinson_system
+    n, nrhs = B.shape
+    dA = torch.from_numpy(A).cuda()
+    dB = torch.from_numpy(np.ascontiguousarray(B)).cuda()
+    s_ = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for max_iter in (0, 5):
+        dX = torch.zeros(n, nrhs, dtype=torch.float64, device="cuda")
+        it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
+        rc = lib.mrm_dev_lineq_solve_ir(_lib.SOLVE_FP64, dA.data_ptr(), n * 8, n, dB.data_ptr(), nrhs * 8,
+                                        dX.data_ptr(), nrhs * 8, nrhs, max_iter, 1e-15, C.byref(it), C.byref(be),
+                                        C.byref(fb), s_)
+        torch.cuda.synchronize()
+        out[max_iter] = (rc, it.value, be.value, dX.cpu().numpy())
+    rc0, it0, be0, X0 = out[0]
+    assert rc0 == _lib.MRM_NOT_CONVERGED and it0 == 0 and be0 > 1e-7
+    assert be0 == pytest.approx(orc.backward_error(A, X0, B), rel=1e-6)
+    rc5, it5, be5, X5 = out[5]
+    assert rc5 == _lib.MRM_OK and 1 <= it5 <= 2 and be5 <= 1e-15
+    assert _pins.forward_error(X5, Xe) <= 2e-15
+
+
+def test_nan_right_hand_side_is_not_converged():
+    """A NaN in b reaches the convergence test (the residual norms carry it):
+    the solve reports MRM_NOT_CONVERGED with a NaN backward error instead of
+    a converged NaN solution."""
+    import ctypes as C
+    import torch
+    lib = _lib.load()
+    n = 300
+    A = torch.from_numpy(orc.pm1_matrix(n, n, 3) + n * np.eye(n)).cuda()
+    Bh = orc.pm1_matrix(2, n, 4)
+    Bh[17, 1] = np.nan
+    B = torch.from_numpy(Bh).cuda()
+    s_ = torch.cuda.current_stream().cuda_stream
+    for mode in (_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR):
+        X = torch.zeros(n, 2, dtype=torch.float64, device="cuda")
+        it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
+        rc = lib.mrm_dev_lineq_solve_ir(mode, A.data_ptr(), n * 8, n, B.data_ptr(), 16, X.data_ptr(), 16, 2, 5,
+                                        1e-14, C.byref(it), C.byref(be), C.byref(fb), s_)
+        assert rc == _lib.MRM_NOT_CONVERGED
+        assert np.isnan(be.value)
+        if mode == _lib.SOLVE_FP32_IR:
+            assert fb.value == 1
+
+
+def test_panel_timeout_relaunches_cooperatively():
+    """A panel whose workgroups are not co-resident times out instead of
+    hanging; the factorisation then runs again from the original matrix with
+    cooperative panel launches.  Forced from the host: deadline 0 makes every
+    wait of a plain launch give up at once (every kernel still exits
+    normally).  Results equal an undisturbed run bit for bit."""
+    import torch
+    lib = _lib.load()
+    n = 2048
+    g = torch.Generator(device="cuda").manual_seed(77)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    s_ = torch.cuda.current_stream().cuda_stream
+
+    def dev_factor():
+        LU = A.clone()
+        indx = torch.zeros(n, dtype=torch.int32, device="cuda")
+        perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+        rc = lib.mrm_dev_lu_factor(LU.data_ptr(), n * 8, n, indx.data_ptr(), perm.data_ptr(), s_)
+        torch.cuda.synchronize()
+        return rc, LU, indx, perm
+
+    rc, LU0, indx0, perm0 = dev_factor()
+    assert rc == 0
+    Ah = orc.pm1_matrix(300, 300, 5)
+    lu_h0 = Ah.copy()
+    ih0 = np.zeros(300, dtype=np.int32)
+    assert la.MatrixLUDecompInPlace(lu_h0, ih0) == la.leOk
+    before = lib.mrm_lu_panel_test_deadline(-1)
+    try:
+        lib.mrm_lu_panel_test_deadline(0)
+        rc, LU1, indx1, perm1 = dev_factor()
+        assert rc == 0
+        assert lib.mrm_lu_panel_test_deadline(0) == before + 1
+        assert torch.equal(LU0, LU1) and torch.equal(indx0, indx1) and torch.equal(perm0, perm1)
+        # host entry: restaged from the caller's matrix
+        lu_h1 = Ah.copy()
+        ih1 = np.zeros(300, dtype=np.int32)
+        assert la.MatrixLUDecompInPlace(lu_h1, ih1) == la.leOk
+        assert lib.mrm_lu_panel_test_deadline(0) == before + 2
+        assert np.array_equal(lu_h0, lu_h1) and np.array_equal(ih0, ih1)
+        # solve entry: refactors its copy of A
+        B = orc.pm1_matrix(2, 300, 6)
+        X = np.zeros((300, 2))
+        assert la.MatrixLinEQSolve(Ah, B, X, 1) == la.leOk
+        assert orc.backward_error(Ah, X, B) <= 1e-14
+        assert lib.mrm_lu_panel_test_deadline(0) == before + 3
+    finally:
+        lib.mrm_lu_panel_test_deadline(-1)
+    # the library stays usable with the default deadline: no more relaunches
+    rc, LU2, _, _ = dev_factor()
+    assert rc == 0 and torch.equal(LU0, LU2)
+    assert lib.mrm_lu_panel_test_deadline(-1) == before + 3
 
 
 def test_backsubst_single_column_matches_oracle():

@@ -159,3 +159,70 @@ def test_host_entry_with_mode(mode):
     assert la.MatrixLinEQSolveEx(A, B, X2, 1, la.smFP64) == la.leOk
     # both are backward stable to 1e-14; forward agreement to kappa * 1e-13
     assert np.linalg.norm(X - X2) / np.linalg.norm(X2) <= 1e3 * 1e-13
+
+
+def test_rows_per_thread_4_above_16384():
+    """n = 16640: the panel kernel packs 4 rows per thread (17 workgroups) and
+    the permutation is built in global memory; FP32-update LU-IR still reaches
+    1e-14 with pivots from every workgroup (rows below 8000 are 100x larger)."""
+    import torch
+    lib = _lib.load()
+    n = 16640
+    g = torch.Generator(device="cuda").manual_seed(16640)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    A[8000:] *= 100.0
+    B = torch.randn(n, 4, dtype=torch.float64, device="cuda", generator=g)
+    X = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+    it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
+    rc = lib.mrm_dev_lineq_solve_ir(_lib.SOLVE_FP32_IR, A.data_ptr(), n * 8, n, B.data_ptr(), 32, X.data_ptr(), 32, 4,
+                                    50, 1e-14, C.byref(it), C.byref(be), C.byref(fb),
+                                    torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    print(f"n={n}: rc={rc} steps={it.value} berr={be.value:.3e} fell_back={fb.value}")
+    assert rc == 0 and fb.value == 0
+    R = B - A @ X
+    berr = (R.abs().amax(dim=0) / (A.abs().sum(dim=1).amax() * X.abs().amax(dim=0) + B.abs().amax(dim=0))).max()
+    assert float(berr) <= 1e-14
+
+
+@pytest.fixture(scope="module")
+def system_16384():
+    """A = U diag(s) V^T (U, V from Householder QR of Gaussian matrices, s
+    log-spaced to kappa = 1e2) and B Gaussian, n = 16384, 64 columns."""
+    import torch
+    n, nrhs = 16384, 64
+    g = torch.Generator(device="cuda").manual_seed(4401)
+    Uq, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    Vq, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    B = torch.randn(n, nrhs, dtype=torch.float64, device="cuda", generator=g)
+    s = torch.logspace(0, -2, n, dtype=torch.float64, device="cuda")
+    A = (Uq * s) @ Vq.t()
+    del Uq, Vq
+    yield A, B
+    del A, B
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR])
+def test_n16384_64rhs_independent_residual(mode, system_16384):
+    """The LU-IR workload itself (n = 16384, 64 right-hand sides, kappa = 1e2):
+    the backward error is recomputed here from an independent FP64 residual
+    (torch's matmul), not taken from the library's statistics."""
+    import torch
+    lib = _lib.load()
+    A, B = system_16384
+    n, nrhs = B.shape
+    X = torch.zeros(n, nrhs, dtype=torch.float64, device="cuda")
+    it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
+    rc = lib.mrm_dev_lineq_solve_ir(mode, A.data_ptr(), n * 8, n, B.data_ptr(), nrhs * 8, X.data_ptr(), nrhs * 8,
+                                    nrhs, 50, 1e-14, C.byref(it), C.byref(be), C.byref(fb),
+                                    torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert rc == 0
+    R = B - A @ X
+    berr = float((R.abs().amax(dim=0) / (A.abs().sum(dim=1).amax() * X.abs().amax(dim=0) +
+                                         B.abs().amax(dim=0))).max())
+    print(f"mode={mode}: steps={it.value} library berr={be.value:.3e} independent berr={berr:.3e} "
+          f"fell_back={fb.value}")
+    assert berr <= 1e-14
+    assert berr <= 4 * be.value + 1e-17

@@ -10,6 +10,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import _pins
 import oracle as orc
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -235,18 +236,62 @@ def test_lineq_solve_matches_numpy(n, nrhs):
     assert orc.backward_error(A, X, B) < 1e-15
 
 
-def test_lineq_solve_refinement_reduces_residual():
-    """NumRefinments > 0 (parity unpinned: no reference test runs it) must not
-    increase the backward error of an ill-conditioned solve."""
-    rng = np.random.default_rng(9)
-    n = 60
-    U, _ = np.linalg.qr(rng.standard_normal((n, n)))
-    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
-    A = U @ np.diag(np.logspace(0, -8, n)) @ V.T
-    B = rng.standard_normal((n, 2))
+@pytest.fixture(scope="module")
+def wilkinson_system():
+    """n = 50 Wilkinson matrix, 3 right-hand sides, and the exact solution."""
+    n = 50
+    A = _pins.wilkinson(n)
+    B = orc.pm1_matrix(3, n, 7)
+    Xe = np.stack([_pins.exact_solve(A, B[:, j]) for j in range(3)], axis=1)
+    return A, B, Xe
+
+
+def test_lineq_solve_refinement_pinned_by_the_exact_solution(wilkinson_system):
+    """MatrixLinEQSolve's refinement loop (src/LinAlgLU.pas:542-586) against
+    the exact rational solution: without refinement the pivot growth 2^49
+    leaves a forward error ~1e-3; one step r = A x - b, x -= (LU)^-1 r
+    restores full accuracy, and further steps keep it.  A loop that does
+    nothing, adds the correction, or reuses the first residual fails."""
+    A, B, Xe = wilkinson_system
+    errs = [_pins.forward_error(orc.lineq_solve(A, B, k)[1], Xe) for k in range(4)]
+    assert errs[0] > 1e-5
+    assert all(e <= 2e-15 for e in errs[1:]), errs
+    # NumRefinments 0 and 1 differ by more than the remaining error
     _, X0 = orc.lineq_solve(A, B, 0)
-    _, X2 = orc.lineq_solve(A, B, 2)
-    assert orc.backward_error(A, X2, B) <= 1.5 * orc.backward_error(A, X0, B) + 1e-16
+    _, X1 = orc.lineq_solve(A, B, 1)
+    assert np.abs(X1 - X0).max() > 1e3 * np.abs(X1 - Xe).max()
+    # backward error: large before, at the FP64 level after
+    assert orc.backward_error(A, X0, B) > 1e-7
+    assert orc.backward_error(A, X1, B) < 1e-16
+
+
+def test_refinement_pin_rejects_planted_mistakes(wilkinson_system):
+    """The pin above fails a wrong-sign correction and a stale residual
+    (both planted here, on the CPU, from the oracle's own LU and residual)."""
+    A, B, Xe = wilkinson_system
+    n = A.shape[0]
+    LU = A.copy()
+    rc, indx = orc.lu_decomp_inplace(LU)
+    assert rc == 0
+    _, X0 = orc.lineq_solve(A, B, 0)
+    R = A @ X0 - B
+    D = R.copy()
+    for j in range(B.shape[1]):
+        orc.lu_backsubst(LU, indx, D, j)
+    assert _pins.forward_error(X0 - D, Xe) <= 2e-15      # the correct step
+    assert _pins.forward_error(X0 + D, Xe) > 1e-5        # wrong sign
+    assert _pins.forward_error(X0 - 2 * D, Xe) > 1e-5    # two steps on the first residual
+    assert n == 50
+
+
+def test_lineq_solve_refinement_per_column(wilkinson_system):
+    """Every right-hand side is refined on its own (the reference loops over
+    the columns): a 3-column solve equals three 1-column solves."""
+    A, B, _ = wilkinson_system
+    _, X = orc.lineq_solve(A, B, 2)
+    for j in range(B.shape[1]):
+        _, xj = orc.lineq_solve(A, np.ascontiguousarray(B[:, j:j + 1]), 2)
+        assert np.array_equal(X[:, j:j + 1], xj)
 
 
 def test_determinant_and_inverse_random():
@@ -265,3 +310,33 @@ def test_backward_error_metric():
     assert orc.backward_error(A, X, B) == 0.0
     X[1, 0] = 1.5  # r = 0.5, ||A||=1, ||x||=1.5, ||b||=1
     assert orc.backward_error(A, X, B) == pytest.approx(0.5 / 2.5)
+
+
+def test_covariance_matches_the_textbook_definition():
+    """Covariance (src/Statistics.pas:599-639): unbiased = numpy.cov, biased
+    = numpy.cov(bias=True), both to rounding."""
+    X = orc.pm1_matrix(9, 200, 31)
+    C = orc.covariance(X, True)
+    assert np.abs(C - np.cov(X, rowvar=False)).max() < 1e-15
+    Cb = orc.covariance(X, False)
+    assert np.abs(Cb - np.cov(X, rowvar=False, bias=True)).max() < 1e-15
+    assert np.array_equal(C, C.T)
+
+
+def test_covariance_closed_forms():
+    """Columns that are exact affine functions of one another: a column plus
+    a constant has the same (co)variance (the mean is removed), 2x a column
+    has 4x the variance and 2x the covariance; integer data gives exact values."""
+    t = np.arange(10.0)
+    X = np.stack([t, t + 100.0, 2 * t, -t], axis=1)
+    C = orc.covariance(X, True)
+    v = float(np.sum((t - t.mean()) ** 2) / 9)  # 9.1666...
+    want = v * np.array([[1, 1, 2, -1], [1, 1, 2, -1], [2, 2, 4, -2], [-1, -1, -2, 1]])
+    assert np.allclose(C, want, rtol=1e-15, atol=0)
+
+
+def test_covariance_single_row_or_column_is_the_variance():
+    """height = 1 or width = 1 gives MatrixVar of the vector (src/Statistics.pas:606-610)."""
+    x = orc.pm1_matrix(1, 17, 3)
+    assert orc.covariance(x, True)[0, 0] == pytest.approx(np.var(x, ddof=1), rel=1e-14)
+    assert orc.covariance(x.T.copy(), False)[0, 0] == pytest.approx(np.var(x), rel=1e-14)

@@ -114,10 +114,10 @@ def test_errors_and_empty():
 
 @pytest.mark.parametrize("trans", [_lib.NN, _lib.TN, _lib.NT])
 def test_device_8192_error_per_slice(trans):
-    """Full size on device tensors: at s = 7 the emulated product agrees with
-    the native FP64 MFMA GEMM (itself checked against the oracle) to the GEMM
-    bar, and sampled rows agree with the oracle GEMM directly; s = 6 is
-    reported."""
+    """Full size on device tensors: at s = 6 (the FP64-accurate operating
+    point of the bench) and s = 7 the emulated product agrees with the native
+    FP64 MFMA GEMM (itself checked against the oracle) to the GEMM bar, and
+    sampled rows agree with the oracle GEMM directly."""
     import torch
     lib = _lib.load()
     n = 8192 if trans == _lib.NN else 4096
@@ -147,6 +147,5 @@ def test_device_8192_error_per_slice(trans):
         err_rows = float(np.linalg.norm(Cr - ref_rows)) / (float(np.linalg.norm(An[rows])) * float(np.linalg.norm(Bn)))
         print(f"trans={trans} n={n} s={s} normwise error vs FP64 GEMM {err:.3e}, sampled rows vs oracle "
               f"{err_rows:.3e}, bar {4 * n * U:.3e}")
-        if s == 7:
-            assert err <= 4 * n * U
-            assert err_rows <= 4 * n * U
+        assert err <= 4 * n * U
+        assert err_rows <= 4 * n * U

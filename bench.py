@@ -7,7 +7,10 @@ of the GPU GEMM family).  Prints one JSON line; besides the headline GEMM it
 carries the other measured rows of the path under "rows":
 
   dgemm_tn_4096 / dgemm_nt_4096   MultT1 / MultT2 at 4096^3
-  syrk_200000x512                 X^T X with X 200000 x 512 (split-K GEMM)
+  syrk_200000x512                 X^T X with X 200000 x 512 (symmetric split-K GEMM),
+                                  also counted against the SYRK minimum n (n + 1) k
+  covariance_200000x512           Covariance of X 200000 x 512: means, centred copy,
+                                  symmetric split-K product, 1/(m-1) scale
   ozaki_8192                      FP64 GEMM emulated with INT8 slices, s = 3..7,
                                   wide-range inputs (2 res53 - 1) 2^e, e in [-20, 20]
   batched                         8192 uneven FP64 GEMMs (m, n, k in [16, 256]) and
@@ -20,8 +23,9 @@ carries the other measured rows of the path under "rows":
   python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-rows]
 
 --workload runs one workload alone (dgemm_nn_8192 default, dgemm_tn_4096,
-dgemm_nt_4096, syrk_200000x512, ozaki_8192, batched, lu_ir_16384, and copy_1gib: a
-plain 1 GiB device copy, the calibration kernel for the HBM byte counters).
+dgemm_nt_4096, syrk_200000x512, covariance_200000x512, ozaki_8192, batched,
+lu_ir_16384, and copy_1gib: a plain 1 GiB device copy, the calibration kernel
+for the HBM byte counters).
 """
 from __future__ import annotations
 
@@ -61,30 +65,70 @@ def _traffic_from_profile(workload: str):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(threads: int, n: int = 2048, budget_s: float = 15.0):
-    """The restated reference fast path (blocked 128 + 4-lane FMA kernel +
-    ThrMatrixMultEx2 tiling) on the host cores, on an n^3 sample."""
+def host_cores() -> int:
+    """CPUs this process may run on (the box's share), at most cMaxNumCores = 64."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(64, n))
+
+
+def host_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:  # pragma: no cover
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n: int = 8192, budget_s: float = 20.0, lu_n: int = 4096, lu_rhs: int = 64):
+    """The restated reference fast path on the host cores: the threaded
+    blocked multiply (block 128, 4-lane FMA kernel, ThrMatrixMultEx2 tiling
+    with NumCoresToUseForMult) on the headline n^3 product, and the restated
+    ThrMatrixLinEQSolve on an lu_n system with lu_rhs right-hand sides.
+    Threads: min(CPUs available, 64) as the reference's pool takes, limited
+    by NumCoresToUseForMult for the multiply."""
     import numpy as np
     import oracle as orc
 
+    threads = host_cores()
     A = orc.pm1_matrix(n, n, 8101)
     B = orc.pm1_matrix(n, n, 8102)
     C = np.zeros((n, n))
     used = orc.fast_mult(C, A, B, threads)  # warm-up
     times = []
     t_start = time.monotonic()
-    while len(times) < 5 and (time.monotonic() - t_start) < budget_s:
+    while len(times) < 3 and (time.monotonic() - t_start) < budget_s:
         t0 = time.monotonic()
         used = orc.fast_mult(C, A, B, threads)
         times.append(time.monotonic() - t0)
     t = statistics.median(times)
+    del A, B, C
+    # LU solve: one warm-up-free run (the factorisation dominates; its run
+    # time is seconds), as the reference's ThrMatrixLinEQSolve would be timed
+    Al = orc.pm1_matrix(lu_n, lu_n, 4401)
+    Bl = orc.pm1_matrix(lu_rhs, lu_n, 4403)
+    t0 = time.monotonic()
+    rc, Xl = orc.thr_lineq_solve(Al, Bl, 0, threads)
+    lu_s = time.monotonic() - t0
+    lu_flops = 2.0 / 3.0 * lu_n ** 3 + 2.0 * lu_n ** 2 * lu_rhs
     return {
         "value": round(2.0 * n ** 3 / t / 1e12, 5),
         "unit": "TFLOP/s",
         "cores": int(used),
         "kind": "port",
-        "sample": f"dgemm NN {n}^3, U[-1,1) MT19937 seeds 8101/8102, median of {len(times)} runs "
-                  f"after 1 warm-up, {used} threads (ThrMatrixMultEx2 tiling, block 128, AVX2/FMA)",
+        "sample": f"dgemm NN {n}^3 (the headline product), U[-1,1) MT19937 seeds 8101/8102, median of "
+                  f"{len(times)} runs after 1 warm-up, {used} threads of {threads} available (ThrMatrixMultEx2 "
+                  f"tiling, block 128, AVX2/FMA kernel)",
+        "host": {"cpu": host_model(), "cpus_available": threads, "nproc": os.cpu_count()},
+        "lu_solve": {"n": lu_n, "rhs": lu_rhs, "ms": round(lu_s * 1e3, 1), "threads": threads, "rc": rc,
+                     "gflops": round(lu_flops / lu_s / 1e9, 2),
+                     "backward_error": orc.backward_error(Al, Xl, Bl),
+                     "what": "restated ThrMatrixLinEQSolve (recursive LU, column-split LUBacksup, threaded "
+                             "blocked Schur update with block 48, threaded MatrixLUBackSubst), one run"},
     }
 
 
@@ -169,6 +213,48 @@ def gemm_run(lib, workload: str, steps: int, warmup: int):
     }
 
 
+def covariance_row(lib, h: int = 200000, w: int = 512, reps: int = 10):
+    """Covariance (src/Statistics.pas:599-639) of X 200000 x 512 on the device
+    entry: column means, centred copy, symmetric split product, scale; HIP
+    events around each call, median."""
+    import torch
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(2001)
+    X = torch.rand(h, w, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    Cv = torch.empty(w, w, dtype=torch.float64, device=dev)
+    wb = lib.mrm_covariance_workspace_bytes(w, h)
+    W = torch.empty(wb, dtype=torch.uint8, device=dev)
+    sp = torch.cuda.current_stream().cuda_stream
+
+    def call():
+        rc = lib.mrm_dev_covariance(X.data_ptr(), w * 8, w, h, Cv.data_ptr(), w * 8, 1, W.data_ptr(), wb, sp)
+        assert rc == 0, rc
+
+    for _ in range(3):
+        call()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ms = statistics.median(ts)
+    # algorithmic traffic: X read twice (means, centring), the centred copy
+    # written once and read once by the product, the w x w result
+    traffic = 8.0 * (4 * h * w + w * w)
+    syrk = w * (w + 1) * h
+    Xc = X - X.mean(dim=0)
+    ref = (Xc[:, :4].t() @ Xc) / (h - 1)
+    err = float(torch.linalg.norm(Cv[:4] - ref) / (torch.linalg.norm(Xc[:, :4]) * torch.linalg.norm(Xc) / (h - 1)))
+    return {"ms": round(ms, 4), "syrk_tflops": round(syrk / (ms * 1e-3) / 1e12, 3),
+            "hbm_gbps_algorithmic": round(traffic / (ms * 1e-3) / 1e9, 1),
+            "sampled_rows_normwise_err": err,
+            "path": "mrm_dev_covariance: colsum/mean/centre kernels + symmetric split-K TN GEMM + scale"}
+
+
 def ozaki_row(lib, reps: int = 3):
     """FP64 GEMM emulated with s = 3..7 INT8 slices at 8192^3 on wide-range
     inputs, against the native FP64 MFMA GEMM of the same operands (itself
@@ -231,7 +317,10 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
     matrices, s log-spaced to kappa), B Gaussian: plain FP64 LU + solve against
     LU with FP32 / BF16 trailing updates and FP64 refinement, all to a normwise
     backward error <= 1e-14.  Times are wall clock around the synchronous
-    device entry (factorisation, solves, residuals, norms), median of 2."""
+    device entry (factorisation, solves, residuals, norms): one untimed call
+    per mode first (it grows the library's work buffers), then the median of
+    3 calls.  The backward error is also recomputed from an independent
+    residual (torch's FP64 matmul)."""
     import ctypes as C
     import torch
 
@@ -251,15 +340,21 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
             X = torch.zeros(n, nrhs, dtype=torch.float64, device=dev)
             it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
             ts = []
-            for _ in range(2):
+            for rep in range(4):
                 torch.cuda.synchronize()
                 t = time.perf_counter()
                 rc = lib.mrm_dev_lineq_solve_ir(mode, A.data_ptr(), n * 8, n, B.data_ptr(), nrhs * 8, X.data_ptr(),
                                                 nrhs * 8, nrhs, 50, 1e-14, C.byref(it), C.byref(be), C.byref(fb), sp)
                 torch.cuda.synchronize()
-                ts.append(time.perf_counter() - t)
+                if rep > 0:
+                    ts.append(time.perf_counter() - t)
+            R = B - A @ X
+            berr = float((R.abs().amax(dim=0) / (A.abs().sum(dim=1).amax() * X.abs().amax(dim=0) +
+                                                 B.abs().amax(dim=0))).max())
+            del R
             res[name] = {"rc": rc, "ms": round(1e3 * statistics.median(ts), 1), "refinement_steps": it.value,
-                         "backward_error": be.value, "fell_back_to_fp64": bool(fb.value)}
+                         "backward_error": be.value, "backward_error_independent": berr,
+                         "fell_back_to_fp64": bool(fb.value)}
         for name in ("fp32_ir", "bf16_ir"):
             res[name]["speedup_vs_fp64"] = round(res["fp64"]["ms"] / res[name]["ms"], 3)
         out["kappa"][f"{kappa:g}"] = res
@@ -336,7 +431,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=None,
-                    choices=sorted(WORKLOADS) + ["ozaki_8192", "lu_ir_16384", "batched", "copy_1gib"])
+                    choices=sorted(WORKLOADS) + ["ozaki_8192", "lu_ir_16384", "batched", "copy_1gib",
+                                                 "covariance_200000x512"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-rows", action="store_true", help="headline GEMM only")
     args = ap.parse_args()
@@ -358,6 +454,9 @@ def main():
         return
     if args.workload == "batched":
         print(json.dumps({"workload": "batched", **batched_row(lib)}), flush=True)
+        return
+    if args.workload == "covariance_200000x512":
+        print(json.dumps({"workload": "covariance_200000x512", **covariance_row(lib)}), flush=True)
         return
     if args.workload == "lu_ir_16384":
         print(json.dumps({"workload": "lu_ir_16384", **lu_ir_row(lib)}), flush=True)
@@ -413,13 +512,17 @@ def main():
                 frac = t * (t + 1) / 2 / (t * t)
                 rows[w]["symmetric_tiles"] = f"{t * (t + 1) // 2}/{t * t}"
                 rows[w]["executed_tflops"] = round(q["tflops"] * frac, 3)
+                # the SYRK minimum n (n + 1) k of a symmetric product
+                syrk = q["M"] * (q["M"] + 1) * q["K"]
+                rows[w]["syrk_tflops"] = round(syrk / (q["ms_per_step"] * 1e-3) / 1e12, 3)
+                rows[w]["syrk_pct_of_fp64_peak"] = round(100 * rows[w]["syrk_tflops"] / FP64_PEAK_TFLOPS, 2)
+        rows["covariance_200000x512"] = covariance_row(lib)
         rows["batched"] = batched_row(lib)
         rows["ozaki_8192"] = ozaki_row(lib)
         rows["lu_ir_16384"] = lu_ir_row(lib)
         out["rows"] = rows
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1, 64)
-        out["cpu_baseline"] = cpu_baseline(threads)
+        out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 
 

@@ -22,7 +22,8 @@ int mrm_launch_dgemm_splitk(int trans, int64_t M, int64_t N, int64_t K, const do
                             double* work, hipStream_t stream);
 
 // The same with the shape's own K split and the partial products in the
-// caller's work (work_bytes checked against what the split needs).
+// caller's work: never allocates; a split that needs more than work_bytes
+// (or any split when work is NULL) is refused with MRM_BAD_DIMS.
 int mrm_launch_dgemm_ws(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                         int64_t ldb, double* C, int64_t ldc, int op, void* work, size_t work_bytes,
                         hipStream_t stream);

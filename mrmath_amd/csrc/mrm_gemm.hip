@@ -466,12 +466,13 @@ Split plan_split(int trans, int64_t M, int64_t N, int64_t K, const void* A, int6
 
 // C (M x N) = / += / -= op(A) * op(B), K the inner dimension.  Strides in bytes.
 // want_splits > 0 asks for that many K chunks (at least BK each), 0 lets the
-// shape decide; work (optional) holds the partial products (used * M * N
-// doubles; when work_bytes > 0 that is checked against it), otherwise they
-// come from a stream-ordered allocation.
+// shape decide.  The partial products of a split (used * M * N doubles) go to
+// work; with checked, work_bytes is what work holds and a split that needs
+// more is an error (nothing is allocated); otherwise work, when given, holds
+// enough, and without it they come from a stream-ordered allocation.
 int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                  int64_t ldb, double* C, int64_t ldc, int op, int want_splits, double* work, size_t work_bytes,
-                 hipStream_t stream) {
+                 bool checked, hipStream_t stream) {
     if (M < 0 || N < 0 || K < 0 || op < 0 || op > 2 || trans < 0 || trans > 2) {
         mrm_set_error(MRM_BAD_ARG, "dgemm: bad argument (M=%lld N=%lld K=%lld op=%d trans=%d)", (long long)M,
                       (long long)N, (long long)K, op, trans);
@@ -507,9 +508,9 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
         const int used = sp.used;
         const int64_t cstride = M * N;
         const size_t need = size_t(used) * size_t(cstride) * 8;
-        if (work && work_bytes > 0 && need > work_bytes) {
+        if (checked && (!work || need > work_bytes)) {
             mrm_set_error(MRM_BAD_DIMS, "dgemm: the split product needs %zu bytes of workspace, %zu given", need,
-                          work_bytes);
+                          work ? work_bytes : size_t(0));
             return MRM_BAD_DIMS;
         }
         void* W = work;
@@ -539,23 +540,20 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
 
 int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                      const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream) {
-    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, nullptr, 0, stream);
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, nullptr, 0, false, stream);
 }
 
 int mrm_launch_dgemm_splitk(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                             const double* B, int64_t ldb, double* C, int64_t ldc, int op, int splits,
                             double* work, hipStream_t stream) {
-    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, splits < 1 ? 1 : splits, work, 0, stream);
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, splits < 1 ? 1 : splits, work, 0, false,
+                        stream);
 }
 
 int mrm_launch_dgemm_ws(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                         int64_t ldb, double* C, int64_t ldc, int op, void* work, size_t work_bytes,
                         hipStream_t stream) {
-    if (!work || work_bytes == 0) {
-        mrm_set_error(MRM_BAD_ARG, "dgemm: no workspace given");
-        return MRM_BAD_ARG;
-    }
-    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, static_cast<double*>(work), work_bytes,
+    return launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, 0, static_cast<double*>(work), work_bytes, true,
                         stream);
 }
 

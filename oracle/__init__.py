@@ -46,6 +46,8 @@ _SIG = {
     "orc_gemm_normwise_err": (C.c_double, [_p, _i64, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64, _i64, _i64]),
     "orc_backward_error": (C.c_double, [_p, _i64, _i64, _p, _i64, _p, _i64, _i64]),
     "orc_covariance": (None, [_p, _i64, _p, _i64, _i64, _i64, _i32]),
+    "orc_thr_lu_decomp": (_i32, [_p, _i64, _i64, _p, C.c_int]),
+    "orc_thr_lineq_solve": (_i32, [_p, _i64, _i64, _p, _i64, _p, _i64, _i64, _i64, C.c_int]),
     "orc_fast_mult": (C.c_int, [_p, _i64, _p, _p, _i64, _i64, _i64, _i64, _i64, _i64, C.c_int, C.c_int]),
     "orc_num_cores_for_mult": (C.c_int, [_i64, _i64, _i64, _i64, C.c_int]),
 }
@@ -225,6 +227,24 @@ def lineq_solve(A: np.ndarray, B: np.ndarray, refinements: int = 0):
     pb, w2, _, lwb = _m(B)
     X = np.zeros((w, w2))
     rc = lib().orc_lineq_solve(pa, lwa, w, pb, lwb, X.ctypes.data, w2 * 8, w2, refinements)
+    return rc, X
+
+
+def thr_lu_decomp(A: np.ndarray, threads: int):
+    """The restated threaded LU (ThrMatrixLUDecomp) in place; (rc, indx)."""
+    pa, w, h, lw = _m(A)
+    assert w == h
+    indx = np.zeros(w, dtype=np.int32)
+    rc = lib().orc_thr_lu_decomp(pa, lw, w, indx.ctypes.data, threads)
+    return rc, indx
+
+
+def thr_lineq_solve(A: np.ndarray, B: np.ndarray, refinements: int = 0, threads: int = 1):
+    """The restated ThrMatrixLinEQSolve (CPU baseline of the solve); (rc, X)."""
+    pa, w, _, lwa = _m(A)
+    pb, w2, _, lwb = _m(B)
+    X = np.zeros((w, w2))
+    rc = lib().orc_thr_lineq_solve(pa, lwa, w, pb, lwb, X.ctypes.data, w2 * 8, w2, refinements, threads)
     return rc, X
 
 

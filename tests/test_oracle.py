@@ -340,3 +340,33 @@ def test_covariance_single_row_or_column_is_the_variance():
     x = orc.pm1_matrix(1, 17, 3)
     assert orc.covariance(x, True)[0, 0] == pytest.approx(np.var(x, ddof=1), rel=1e-14)
     assert orc.covariance(x.T.copy(), False)[0, 0] == pytest.approx(np.var(x), rel=1e-14)
+
+
+@pytest.mark.parametrize("n,threads", [(200, 1), (500, 4), (700, 8)])
+def test_threaded_lu_matches_the_recursive_lu(n, threads):
+    """The restated ThrMatrixLUDecomp (src/LinAlgLU.pas:674-803) is the
+    recursive LU with threaded parts: same pivots as the oracle's LU, factors
+    equal to the rounding of the different multiply kernel."""
+    A = orc.pm1_matrix(n, n, 300 + n)
+    A1, A2 = A.copy(), A.copy()
+    rc1, i1 = orc.lu_decomp_inplace(A1)
+    rc2, i2 = orc.thr_lu_decomp(A2, threads)
+    assert rc1 == rc2 == 0
+    assert np.array_equal(i1, i2)
+    assert np.abs(A1 - A2).max() <= 64 * n * 2.0 ** -53 * np.abs(A).max()
+
+
+def test_threaded_lineq_solve_and_refinement(wilkinson_system):
+    """ThrMatrixLinEQSolve (src/LinAlgLU.pas:986-1128): backward-stable on a
+    random system with threads, and its refinement passes the exact-solution
+    pin of the Wilkinson system."""
+    n = 600
+    A = orc.pm1_matrix(n, n, 61)
+    B = orc.pm1_matrix(7, n, 62)
+    rc, X = orc.thr_lineq_solve(A, B, 0, 6)
+    assert rc == 0 and orc.backward_error(A, X, B) <= 1e-14
+    Aw, Bw, Xe = wilkinson_system
+    errs = [_pins.forward_error(orc.thr_lineq_solve(Aw, Bw, k, 3)[1], Xe) for k in range(3)]
+    assert errs[0] > 1e-5 and errs[1] <= 2e-15 and errs[2] <= 2e-15
+    S = np.array([[1.0, 2.0], [2.0, 4.0]])
+    assert orc.thr_lineq_solve(S, np.ones((2, 1)), 0, 2)[0] == 1  # leSingular

@@ -143,6 +143,22 @@ MRM_API void mrm_blocked_mult_t2(double* dest, int64_t destLineWidth, const doub
 MRM_API int32_t mrm_dev_dgemm(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                       const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, void* stream);
 
+/* Bytes of the device array mrm_dev_dgemm_noalloc keeps its partial products in. */
+#define MRM_NOALLOC_SCRATCH_BYTES (128ll << 20)
+
+/* mrm_dev_dgemm without any allocation, for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  Same arguments, kernels and K
+ * split as mrm_dev_dgemm, so the result is the same bit for bit; a split
+ * product (few output tiles, long K: X^T X) keeps its partial products in a
+ * library-owned device array of MRM_NOALLOC_SCRATCH_BYTES per GPU instead of
+ * a stream-ordered allocation.  Every split fits (at most 128 MiB); a larger
+ * need would be refused with MRM_BAD_DIMS.  All calls on one GPU share that
+ * array: calls that overlap in time on one GPU race. */
+MRM_API int32_t mrm_dev_dgemm_noalloc(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A,
+                                      int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op,
+                                      void* stream);
+
 /* ---- threaded callers: one device GEMM instead of up to 64 tile calls ---- */
 
 /* ThrMatrixMult (reference src/ThreadedMatrixOperations.pas:729-735, the body

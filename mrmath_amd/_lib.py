@@ -51,6 +51,7 @@ SIGNATURES = {
     "mrm_blocked_mult_t1": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _pd]),
     "mrm_blocked_mult_t2": (None, [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _pd]),
     "mrm_dev_dgemm": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _pv]),
+    "mrm_dev_dgemm_noalloc": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _pv]),
     "mrm_ozaki_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32]),
     "mrm_dev_dgemm_ozaki": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _i32, _pv, _i64,
                                    _pv]),

@@ -516,6 +516,8 @@ def main():
                 syrk = q["M"] * (q["M"] + 1) * q["K"]
                 rows[w]["syrk_tflops"] = round(syrk / (q["ms_per_step"] * 1e-3) / 1e12, 3)
                 rows[w]["syrk_pct_of_fp64_peak"] = round(100 * rows[w]["syrk_tflops"] / FP64_PEAK_TFLOPS, 2)
+                # HBM bytes per launch from the committed counter summary
+                rows[w]["traffic"] = _traffic_from_profile(w)
         rows["covariance_200000x512"] = covariance_row(lib)
         rows["batched"] = batched_row(lib)
         rows["ozaki_8192"] = ozaki_row(lib)

@@ -11,12 +11,13 @@
 //     from an atomic counter (largest K first).  T in {32, 64, 128} is picked
 //     by timing the actual batch (the autotuner) unless the caller fixes it.
 //   LU solve: systems are grouped by size class (n <= 32, 64, 96, 128); per class
-//     one persistent launch where each workgroup (one wave up to 64 unknowns,
-//     four above) takes whole systems from a counter,
-//     holds A in LDS, factors it with partial pivoting (the reference's rule:
-//     first row of maximal |a|, singular at |pivot| <= 10/MaxDouble) and
-//     substitutes.  FP64, or FP32 factors with FP64 residuals and refinement
-//     (x += (LU)^-1 (b - A x), A read again in FP64 from memory).
+//     one persistent launch where each wave (up to 64 unknowns) or each
+//     workgroup of eight waves (above) takes whole systems from a counter,
+//     holds A in registers, factors it with partial pivoting (the reference's
+//     rule: first row of maximal |a|, singular at |pivot| <= 10/MaxDouble) and
+//     substitutes (mrm_batched_lu_wave.hip, mrm_batched_lu_block.h).  FP64,
+//     or FP32 factors with FP64 residuals and refinement (x += (LU)^-1 (b - A x),
+//     A read again in FP64 from memory).
 // The counter reset and the launches are captured once into a hipGraph; a run
 // is one hipGraphLaunch.
 
@@ -27,8 +28,8 @@
 #include <type_traits>
 #include <vector>
 
+#include "mrm_batched_lu.h"
 #include "mrm_common.h"
-#include "mrm_wave.h"
 #include "mrm_host.h"
 
 namespace {
@@ -151,214 +152,8 @@ __global__ void __launch_bounds__(BG_THREADS) bg_gemm_kernel(const GemmP* __rest
     }
 }
 
-// ---------------------------------------------------------------------------
-// Batched small LU solves: one wave per system, A in LDS (row stride n+1).
-// ---------------------------------------------------------------------------
-struct LuP {  // element stride
-    int64_t n, lda;
-    const double* A;
-    const double* b;
-    double* x;
-    int idx;  // position in the caller's list (status output)
-};
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("" ::: "memory");
-}
-
-template <typename T>
-__device__ __forceinline__ T tiny_pivot();
-template <>
-__device__ __forceinline__ double tiny_pivot<double>() { return 10.0 / DBL_MAX; }
-template <>
-__device__ __forceinline__ float tiny_pivot<float>() { return 10.0f / FLT_MAX; }
-
-// Block-wide sync for NWV waves (NWV == 1: the wave's own LDS ordering).
-template <int NWV>
-__device__ __forceinline__ void blk_sync() {
-    if constexpr (NWV == 1) wave_sync();
-    else __syncthreads();
-}
-
-// LU with partial pivoting of the n x n matrix L (row stride ld) in LDS by NWV
-// waves; rows are exchanged physically and perm follows them (row i of P*A is
-// row perm[i] of A).  Returns false when singular.  wv / wr: NWV slots of LDS
-// scratch.
-template <typename T, int NWV>
-__device__ bool block_lu(T* L, int ld, int n, int* perm, int tid, T* wv, int* wr) {
-    constexpr int NT = 64 * NWV;
-    const int lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < n; i += NT) perm[i] = i;
-    blk_sync<NWV>();
-    for (int j = 0; j < n; ++j) {
-        // first row of maximal |a| in column j
-        T best = T(-1);
-        int brow = n;
-        for (int i = j + tid; i < n; i += NT) {
-            const T v = fabs(L[i * ld + j]);
-            if (v > best) { best = v; brow = i; }
-        }
-        mrm::wave::argmax(best, brow);
-        if constexpr (NWV > 1) {
-            if (lane == 0) {
-                wv[wave] = best;
-                wr[wave] = brow;
-            }
-            __syncthreads();
-            best = wv[0];
-            brow = wr[0];
-#pragma unroll
-            for (int q = 1; q < NWV; ++q)
-                if (wv[q] > best || (wv[q] == best && wr[q] < brow)) { best = wv[q]; brow = wr[q]; }
-        }
-        if (!(best > tiny_pivot<T>())) return false;  // uniform
-        if (brow != j && tid == 0) {
-            const int pj = perm[j];
-            perm[j] = perm[brow];
-            perm[brow] = pj;
-        }
-        if (brow != j)
-            for (int k = tid; k < n; k += NT) {
-                const T t = L[j * ld + k];
-                L[j * ld + k] = L[brow * ld + k];
-                L[brow * ld + k] = t;
-            }
-        blk_sync<NWV>();
-        const T rinv = T(1) / L[j * ld + j];
-        for (int i = j + 1 + tid; i < n; i += NT) L[i * ld + j] *= rinv;
-        blk_sync<NWV>();
-        // thread owns column k = j+1+lane (+64) and the rows i = j+1+wave
-        // (+NWV...); L[i][j] is the same for all lanes of a wave (broadcast).
-        // Rows go in groups of 8 whose loads are issued before the stores.
-        for (int k = j + 1 + lane; k < n; k += 64) {
-            const T ujk = L[j * ld + k];
-            for (int i0 = j + 1 + wave; i0 < n; i0 += 8 * NWV) {
-                T lij[8], lik[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int i = i0 + u * NWV < n ? i0 + u * NWV : n - 1;
-                    lij[u] = L[i * ld + j];
-                    lik[u] = L[i * ld + k];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (i0 + u * NWV < n) L[(i0 + u * NWV) * ld + k] = lik[u] - lij[u] * ujk;
-            }
-        }
-        blk_sync<NWV>();
-    }
-    return true;
-}
-
-// y := (LU)^-1 P y in place (y in LDS, type Y, n <= 128), with the factors
-// and permutation of block_lu.  P y is one gather.
-template <typename T, typename Y>
-__device__ void wave_lu_subst(const T* L, int ld, int n, const int* perm, Y* y, int lane) {
-    const Y g0 = lane < n ? y[perm[lane]] : Y(0);
-    const Y g1 = lane + 64 < n ? y[perm[lane + 64]] : Y(0);
-    wave_sync();
-    if (lane < n) y[lane] = g0;
-    if (lane + 64 < n) y[lane + 64] = g1;
-    wave_sync();
-    for (int k = 0; k < n; ++k) {            // unit lower
-        const Y yk = y[k];
-        for (int i = k + 1 + lane; i < n; i += 64) y[i] -= Y(L[i * ld + k]) * yk;
-        wave_sync();
-    }
-    for (int k = n - 1; k >= 0; --k) {       // upper
-        if (lane == 0) y[k] = y[k] / Y(L[k * ld + k]);
-        wave_sync();
-        const Y yk = y[k];
-        for (int i = lane; i < k; i += 64) y[i] -= Y(L[i * ld + k]) * yk;
-        wave_sync();
-    }
-}
-
-// MODE 0: FP64 factors.  MODE 1: FP32 factors, `refine` FP64 refinement steps.
-// NWV waves per system (the factorisation is shared by all, the
-// substitutions run on wave 0).  Dynamic LDS per workgroup: nmax*(nmax+1)
-// factors of type T, 2*nmax doubles, nmax ints.
-template <int MODE, int NWV>
-__global__ void __launch_bounds__(64 * NWV) bl_solve_kernel(const LuP* __restrict__ sys, int count, int nmax,
-                                                            unsigned* __restrict__ counter, int* __restrict__ status,
-                                                            int refine) {
-    using T = typename std::conditional<MODE == 0, double, float>::type;
-    constexpr int NT = 64 * NWV;
-    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    __shared__ T wv[NWV];
-    __shared__ int wr[NWV];
-    __shared__ int s_item;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    T* L = reinterpret_cast<T*>(lds);
-    double* xv = reinterpret_cast<double*>(lds + ((size_t(nmax) * (nmax + 1) * sizeof(T) + 15) & ~size_t(15)));
-    double* rv = xv + nmax;
-    int* perm = reinterpret_cast<int*>(rv + nmax);
-    for (;;) {
-        if (tid == 0) s_item = int(atomicAdd(counter, 1u));
-        __syncthreads();
-        const int item = s_item;
-        __syncthreads();
-        if (item >= count) return;
-        const LuP s = sys[item];
-        const int n = int(s.n), ld = n + 1;
-        // A into LDS, 16 rows (per wave) x 2 columns per lane in flight per
-        // group: the loads of a group are issued before any of its LDS stores
-        for (int r0 = wave * 16; r0 < n; r0 += 16 * NWV) {
-            double a[16][2];
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = r0 + u, c = lane + 64 * h;
-                    a[u][h] = (r < n && c < n) ? s.A[int64_t(r) * s.lda + c] : 0.0;
-                }
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = r0 + u, c = lane + 64 * h;
-                    if (r < n && c < n) L[r * ld + c] = T(a[u][h]);
-                }
-        }
-        for (int i = tid; i < n; i += NT) xv[i] = s.b[i];
-        __syncthreads();
-        const bool ok = block_lu<T, NWV>(L, ld, n, perm, tid, wv, wr);
-        if (ok) {
-            if (MODE == 0) {
-                if (wave == 0) wave_lu_subst<T, double>(L, ld, n, perm, xv, lane);
-                __syncthreads();
-            } else {
-                // FP32 solve of the FP64 right-hand side, then FP64 refinement
-                float* yv = reinterpret_cast<float*>(rv);
-                for (int i = tid; i < n; i += NT) yv[i] = float(xv[i]);
-                __syncthreads();
-                if (wave == 0) wave_lu_subst<T, float>(L, ld, n, perm, yv, lane);
-                __syncthreads();
-                for (int i = tid; i < n; i += NT) xv[i] = double(yv[i]);
-                __syncthreads();
-                for (int it = 0; it < refine; ++it) {
-                    // r = b - A x with A and b from memory in FP64 (row i per thread)
-                    for (int i = tid; i < n; i += NT) {
-                        double r = s.b[i];
-                        const double* arow = s.A + int64_t(i) * s.lda;
-                        for (int k = 0; k < n; ++k) r -= arow[k] * xv[k];
-                        yv[i] = float(r);  // threads own disjoint rows
-                    }
-                    __syncthreads();
-                    if (wave == 0) wave_lu_subst<T, float>(L, ld, n, perm, yv, lane);
-                    __syncthreads();
-                    for (int i = tid; i < n; i += NT) xv[i] += double(yv[i]);
-                    __syncthreads();
-                }
-            }
-            for (int i = tid; i < n; i += NT) s.x[i] = xv[i];
-        }
-        if (tid == 0 && status) status[s.idx] = ok ? MRM_OK : MRM_SINGULAR;
-        __syncthreads();
-    }
-}
+// Batched small LU solves: the kernels of mrm_batched_lu.h.
+using LuP = mrm::BatchLuP;
 
 // ---------------------------------------------------------------------------
 // Plans
@@ -432,34 +227,26 @@ int launch_gemm(Plan& pl, int t, hipStream_t s) {
     return MRM_OK;
 }
 
-size_t lu_lds(int nmax, int mode) {
-    const size_t fac = size_t(nmax) * (nmax + 1) * (mode == 0 ? 8 : 4);
-    return ((fac + 15) & ~size_t(15)) + size_t(nmax) * 16 + size_t(nmax) * 4;
-}
-
-// Systems up to 64 unknowns get one wave each; larger ones four waves, which
-// share the factorisation (their matrix alone fills most of a CU's LDS).
-int lu_block(int cls) { return cls < 2 ? 64 : 256; }
-
-const void* lu_fn(int mode, int block) {
+// Systems up to 64 unknowns go one per wave (four per workgroup), larger
+// ones one per workgroup of eight waves (mrm_batched_lu_*.hip).
+const void* lu_fn(int mode, int cls) {
+    static const int nmax[Plan::NCLS] = {32, 64, 96, 128};
+    if (cls < 2) return mrm::batched_lu_wave_fn(mode, nmax[cls]);
     if (mode == 0)
-        return block == 64 ? reinterpret_cast<const void*>(&bl_solve_kernel<0, 1>)
-                           : reinterpret_cast<const void*>(&bl_solve_kernel<0, 4>);
-    return block == 64 ? reinterpret_cast<const void*>(&bl_solve_kernel<1, 1>)
-                       : reinterpret_cast<const void*>(&bl_solve_kernel<1, 4>);
+        return cls == 2 ? mrm::batched_lu_block_kernel<0, 96>() : mrm::batched_lu_block_kernel<0, 128>();
+    return cls == 2 ? mrm::batched_lu_block_kernel<1, 96>() : mrm::batched_lu_block_kernel<1, 128>();
 }
 
 int launch_lu(Plan& pl, hipStream_t s) {
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, pl.counters, Plan::NCLS);
     for (int c = 0; c < Plan::NCLS; ++c) {
         if (pl.counts[c] == 0) continue;
-        const int block = lu_block(c);
-        const void* fn = lu_fn(pl.mode, block);
-        const size_t lds = lu_lds(pl.nmax[c], pl.mode);
-        const int grid = resident_grid(fn, block, lds);
+        const void* fn = lu_fn(pl.mode, c);
+        const int block = c < 2 ? mrm::BL_THREADS : mrm::BL_BLOCK_THREADS;
+        const int grid = resident_grid(fn, block, 0);
         unsigned* ctr = pl.counters + c;
-        void* args[] = {&pl.sys[c], &pl.counts[c], &pl.nmax[c], &ctr, &pl.status, &pl.refine};
-        MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds, s));
+        void* args[] = {&pl.sys[c], &pl.counts[c], &ctr, &pl.status, &pl.refine};
+        MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s));
     }
     return MRM_OK;
 }
@@ -633,11 +420,6 @@ int32_t mrm_batched_lu_plan(const mrm_lu_desc* systems, int64_t count, int32_t m
             rc = MRM_HIP_ERROR;
         off += align256(cls[c].size() * sizeof(LuP) + 1);
     }
-    for (int m : {0, 1})
-        for (int block : {64, 256})
-            if (hipFuncSetAttribute(lu_fn(m, block), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(lu_lds(128, m))) != hipSuccess)
-                rc = MRM_HIP_ERROR;
     pl->tile = 64;
     if (!rc) rc = capture(*pl, s);
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MRM_HIP_ERROR;

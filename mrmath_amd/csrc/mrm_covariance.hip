@@ -26,44 +26,69 @@ constexpr int COV_ROWS = 512;   // rows per partial column sum
 
 int64_t row_blocks(int64_t height) { return (height + COV_ROWS - 1) / COV_ROWS; }
 
-// part[b][c] = sum of X[r][c] over the rows r of block b, in row order; one
-// thread per column, so each row is read by consecutive threads (coalesced).
+// part[b][c] = sum of X[r][c] over the rows r of row block b: wave q of the
+// workgroup sums rows q, q+4, q+8, ... of the block for 64 consecutive columns
+// (one per lane: each row segment is one coalesced 512-byte read), then the
+// four wave sums are added in wave order.  Fixed order, so deterministic.
 __global__ void __launch_bounds__(COV_THREADS) colsum_partial_kernel(const double* __restrict__ X, int64_t ldx,
                                                                     int64_t width, int64_t height,
                                                                     double* __restrict__ part) {
-    const int64_t c = int64_t(blockIdx.x) * COV_THREADS + threadIdx.x;
+    __shared__ double ws[4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int64_t c = int64_t(blockIdx.x) * 64 + lane;
     const int64_t b = blockIdx.y;
-    if (c >= width) return;
     const int64_t r0 = b * COV_ROWS;
     const int64_t r1 = r0 + COV_ROWS < height ? r0 + COV_ROWS : height;
     double s = 0.0;
-    for (int64_t r = r0; r < r1; ++r) s += X[r * ldx + c];
-    part[b * width + c] = s;
+    if (c < width) {
+        const double* p = X + c;
+        int64_t r = r0 + q;
+        // four independent loads in flight per step, summed in row order
+        for (; r + 12 < r1; r += 16) {
+            const double a0 = p[r * ldx], a1 = p[(r + 4) * ldx], a2 = p[(r + 8) * ldx], a3 = p[(r + 12) * ldx];
+            s += a0;
+            s += a1;
+            s += a2;
+            s += a3;
+        }
+        for (; r < r1; r += 4) s += p[r * ldx];
+    }
+    ws[q][lane] = s;
+    __syncthreads();
+    if (q == 0 && c < width) part[b * width + c] = ((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane];
 }
 
-// mean[c] = (sum over the partials of column c, in block order) / height
-// (GenericMtxMean divides the column sum by the height,
-// src/SimpleMatrixOperations.pas:432-445).
+// mean[c] = (sum over the partials of column c) / height (GenericMtxMean
+// divides the column sum by the height, src/SimpleMatrixOperations.pas:432-445);
+// wave q sums partials q, q+4, ..., the four sums are added in wave order.
 __global__ void __launch_bounds__(COV_THREADS) colmean_kernel(const double* __restrict__ part, int64_t width,
                                                              int64_t blocks, int64_t height,
                                                              double* __restrict__ mean) {
-    const int64_t c = int64_t(blockIdx.x) * COV_THREADS + threadIdx.x;
-    if (c >= width) return;
+    __shared__ double ws[4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int64_t c = int64_t(blockIdx.x) * 64 + lane;
     double s = 0.0;
-    for (int64_t b = 0; b < blocks; ++b) s += part[b * width + c];
-    mean[c] = s / double(height);
+    if (c < width)
+        for (int64_t b = q; b < blocks; b += 4) s += part[b * width + c];
+    ws[q][lane] = s;
+    __syncthreads();
+    if (q == 0 && c < width) mean[c] = (((ws[0][lane] + ws[1][lane]) + ws[2][lane]) + ws[3][lane]) / double(height);
 }
 
 // ac = X - mean (MatrixSubVec row-wise, src/Statistics.pas:625); ac is dense
-// (stride = width).  Two doubles per thread when the rows allow it.
+// (stride = width).  Two doubles per thread.
 __global__ void __launch_bounds__(COV_THREADS) centre_kernel(const double* __restrict__ X, int64_t ldx,
                                                             int64_t width, int64_t height,
                                                             const double* __restrict__ mean,
                                                             double* __restrict__ ac) {
+    const int64_t half = (width + 1) / 2;  // column pairs per row
     const int64_t idx = int64_t(blockIdx.x) * COV_THREADS + threadIdx.x;
-    if (idx >= width * height) return;
-    const int64_t r = idx / width, c = idx - r * width;
-    ac[idx] = X[r * ldx + c] - mean[c];
+    if (idx >= half * height) return;
+    const int64_t r = idx / half, c = 2 * (idx - r * half);
+    const double* x = X + r * ldx + c;
+    double* d = ac + r * width + c;
+    d[0] = x[0] - mean[c];
+    if (c + 1 < width) d[1] = x[1] - mean[c + 1];
 }
 
 // C = C * scale + 0 (MatrixScaleAndAdd(dest, ..., 0, 1/m), src/Statistics.pas:634;
@@ -101,11 +126,12 @@ int covariance_device(const double* X, int64_t ldx, int64_t width, int64_t heigh
     double* part = reinterpret_cast<double*>(work + L.part);
     double* mean = reinterpret_cast<double*>(work + L.mean);
     const int64_t blocks = row_blocks(height);
-    hipLaunchKernelGGL(colsum_partial_kernel, dim3(grid(width), unsigned(blocks)), dim3(COV_THREADS), 0, s, X, ldx,
+    const unsigned col_groups = unsigned((width + 63) / 64);
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3(col_groups, unsigned(blocks)), dim3(COV_THREADS), 0, s, X, ldx,
                        width, height, part);
-    hipLaunchKernelGGL(colmean_kernel, dim3(grid(width)), dim3(COV_THREADS), 0, s, part, width, blocks, height, mean);
-    hipLaunchKernelGGL(centre_kernel, dim3(grid(width * height)), dim3(COV_THREADS), 0, s, X, ldx, width, height,
-                       mean, ac);
+    hipLaunchKernelGGL(colmean_kernel, dim3(col_groups), dim3(COV_THREADS), 0, s, part, width, blocks, height, mean);
+    hipLaunchKernelGGL(centre_kernel, dim3(grid((width + 1) / 2 * height)), dim3(COV_THREADS), 0, s, X, ldx, width,
+                       height, mean, ac);
     MRM_HIP_TRY(hipGetLastError());
     // ac^T ac: both operands are ac, so the symmetric path computes the upper tiles
     int rc = mrm_launch_dgemm_ws(MRM_TN, width, width, height, ac, width * 8, ac, width * 8, C, ldc * 8, MRM_DO_NONE,

@@ -17,11 +17,12 @@
 // column (B) l%32 and one 16-byte k chunk chosen by l/32; the result register
 // v of lane l is row 8(v/4) + 4(l/32) + v%4, column l%32).
 //
-// Slices are stored in 64-byte k blocks, [slice][k/64][row][k%64], K padded to
-// a multiple of 64 and rows to a multiple of the 256-row tile with zeros, for
-// A by rows and for B by columns: one K step of a 256-row panel is 16 KiB of
+// Slices are stored in 32-byte k blocks, [slice][k/32][row][k%32], K padded to
+// a multiple of 64 and rows to a multiple of the 128-row tile with zeros, for
+// A by rows and for B by columns: one K step of a 128-row panel is 4 KiB of
 // contiguous memory, and the GEMM kernel has no edge cases except in its
-// epilogue.
+// epilogue.  One launch forms the slice products of up to three diagonals
+// (oz_fused_kernel).
 
 #include <cmath>
 #include <cstdint>
@@ -37,7 +38,8 @@ typedef int mrm_i16 __attribute__((ext_vector_type(16)));
 typedef const __attribute__((address_space(1))) void* gptr_t;
 typedef __attribute__((address_space(3))) void* lptr_t;
 
-constexpr int OZ_BK = 64;                 // bytes (= int8 k-values) per K step
+constexpr int OZ_KB = 32;                 // bytes (= int8 k-values) per k block of the slice layout
+constexpr int OZ_KPAD = 64;               // K is padded to a multiple of this
 constexpr int OZ_THREADS = 256;
 
 // ---------------------------------------------------------------------------
@@ -97,8 +99,8 @@ __global__ void __launch_bounds__(256) oz_exp_cols_kernel(const double* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Slicing: a 64 (r) x 64 (k) tile per workgroup; S[t][k/64][r][k%64] (k-block
-// pitch Rp*64 bytes, slice pitch spitch bytes).  Padding rows / k beyond R / K
+// Slicing: a 64 (r) x 64 (k) tile per workgroup; S[t][k/32][r][k%32] (k-block
+// pitch Rp*32 bytes, slice pitch spitch bytes).  Padding rows / k beyond R / K
 // are written as 0.  Each thread cuts 4 consecutive k of one row and stores
 // their 4 digits per slice as one 32-bit word.
 // ---------------------------------------------------------------------------
@@ -133,200 +135,277 @@ __global__ void __launch_bounds__(256) oz_slice_kernel(const double* __restrict_
         }
     }
     __syncthreads();
-    // 64 rows x 64 bytes per slice: one contiguous 4 KiB block
+    // per slice two k blocks of 64 rows x 32 bytes, each one contiguous 2 KiB run
     for (int q = threadIdx.x; q < s * 64 * 4; q += 256) {
-        const int t = q / 256, rr = (q / 4) % 64, c = q % 4;
+        const int t = q / 256, ch = q & 1, rr = (q / 2) % 64, kb = (q / 128) % 2;
         const int64_t r = r0 + rr;
-        const int4 v = *reinterpret_cast<const int4*>(&tile[t][rr][16 * c]);
-        *reinterpret_cast<int4*>(S + t * spitch + (k0 / 64) * Rp * 64 + r * 64 + 16 * c) = v;
+        const int4 v = *reinterpret_cast<const int4*>(&tile[t][rr][32 * kb + 16 * ch]);
+        *reinterpret_cast<int4*>(S + t * spitch + (k0 / OZ_KB + kb) * Rp * OZ_KB + r * OZ_KB + 16 * ch) = v;
     }
 }
 
 // ---------------------------------------------------------------------------
-// INT8 GEMM over one diagonal d: acc = sum_{t=0..d} A_t * B_(d-t)^T (exact
-// int32), then C (op)= acc * 2^(ea_i + eb_j - 7(d+2)).
-// Workgroup tile BM x BN = 64 PM x 64 PN, K step 64 bytes; 4 waves in 2 x 2,
-// each PM x PN blocks of 32 x 32 (v_mfma_i32_32x32x32_i8, which measures
-// ~4.5 POPS against ~3.5 for the 16x16x64 form).  S-stage LDS ring
-// filled by direct global->LDS loads, one barrier per K step.
-// LDS panels [row][64 bytes], 16-byte chunk c of row r at slot c ^ ((r/4) % 4).
+// Several diagonals in one launch: diagonals D0 .. D0+ND-1 of one K chunk.
+// Every pair (t, u) with D0 <= t + u < D0 + ND uses slices t, u < T = D0 + ND
+// of either side, so one K step loads the T slice panels of A and of B once
+// and forms all the pairs from LDS, with one int32 accumulator set per
+// diagonal (exact: |sum| <= (d+1) KCHUNK 127^2 < 2^31).  Workgroup tile 128 x 128, four
+// waves in 2 x 2 of 64 x 64 (2 x 2 blocks of 32 x 32); K step 32 bytes (one
+// v_mfma_i32_32x32x32_i8 k extent), NSTAGE-stage LDS ring
+// [stage][A slices, B slices][128 rows][32 B], the 16-byte chunk c of row r at
+// slot c ^ ((r/8) % 2) so a fragment read of 16 consecutive rows covers all
+// banks.  The epilogue adds the ND diagonals into C in order, with the same
+// FP64 roundings as one launch per diagonal.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int oz_slot(int r, int c) { return c ^ ((r >> 2) & 3); }
-
-// XCD-aware tile order: consecutive workgroups go to different XCDs, so XCD x
-// gets a contiguous run of logical tiles, walked in groups of 4 tile rows.
-__device__ __forceinline__ void oz_tile_coords(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
-    constexpr int NXCD = 8, GROUP_M = 4;
-    const int total = tiles_m * tiles_n;
-    const int xcd = bid % NXCD;
-    const int q = total / NXCD, rem = total % NXCD;
-    const int logical = xcd * q + (xcd < rem ? xcd : rem) + bid / NXCD;
-    const int per_group = GROUP_M * tiles_n;
-    const int group = logical / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = (tiles_m - first_m) < GROUP_M ? (tiles_m - first_m) : GROUP_M;
-    const int in_group = logical - group * per_group;
-    tm = first_m + in_group % gsize;
-    tn = in_group / gsize;
+constexpr bool oz_pair(int D0, int ND, int t, int u) { return t + u >= D0 && t + u < D0 + ND; }
+// smallest UH with at least half of the group's pairs at u < UH
+constexpr int oz_split_u(int D0, int ND) {
+    const int T = D0 + ND;
+    int total = 0;
+    for (int u = 0; u < T; ++u)
+        for (int t = 0; t < T; ++t) total += oz_pair(D0, ND, t, u) ? 1 : 0;
+    int acc = 0;
+    for (int u = 0; u < T; ++u) {
+        for (int t = 0; t < T; ++t) acc += oz_pair(D0, ND, t, u) ? 1 : 0;
+        if (2 * acc >= total) return u + 1;
+    }
+    return T;
+}
+// slice t of A takes part in a pair of half h (0: u < UH, 1: u >= UH)
+constexpr bool oz_a_used(int D0, int ND, int UH, int h, int t) {
+    for (int u = h ? UH : 0; u < (h ? D0 + ND : UH); ++u)
+        if (oz_pair(D0, ND, t, u)) return true;
+    return false;
 }
 
-template <int PM, int PN, int S>
-__global__ void __launch_bounds__(OZ_THREADS, (PM * PN > 8 ? 1 : 2))
-oz_gemm_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, int64_t kbpa, int64_t kbpb, int64_t spa, int64_t spb,
-               int64_t kstep0, int64_t nk, int d, const int* __restrict__ ea, const int* __restrict__ eb,
-               double* __restrict__ C, int64_t ldc, int64_t M, int64_t N, int op, int tiles_m, int tiles_n) {
-    constexpr int BM = 64 * PM, BN = 64 * PN;
-    constexpr int LA = BM / 64, LB = BN / 64;  // 16-row load instructions per wave
-    constexpr int STAGE = (BM + BN) * OZ_BK;
-    __shared__ __attribute__((aligned(16))) int8_t smem[S * STAGE];
+template <int D0, int ND, int NSTAGE>
+__global__ void __launch_bounds__(OZ_THREADS, 1)
+oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, int64_t kbpa, int64_t kbpb, int64_t spa,
+                int64_t spb, int64_t kstep0, int64_t nk, const int* __restrict__ ea, const int* __restrict__ eb,
+                double* __restrict__ C, int64_t ldc, int64_t M, int64_t N, int op, int tiles_m, int tiles_n) {
+    constexpr int T = D0 + ND;              // slices 0..T-1 of each operand take part
+    constexpr int RB = OZ_KB;               // bytes of k per row and step
+    constexpr int PANEL = 128 * RB;         // one slice of one operand, one step
+    constexpr int STAGE = 2 * T * PANEL;
+    __shared__ __attribute__((aligned(16))) int8_t smem[NSTAGE * STAGE];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     int tm, tn;
-    oz_tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-    const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+    {
+        // workgroups are dealt round-robin to the 8 XCDs: XCD x gets a
+        // contiguous run of logical tiles, walked in groups of 8 tile rows so
+        // the workgroups resident on one XCD share slice panels in its L2
+        constexpr int NXCD = 8, GROUP_M = 8;
+        const int bid = blockIdx.x, total = tiles_m * tiles_n;
+        const int xcd = bid % NXCD;
+        const int q = total / NXCD, rem = total % NXCD;
+        const int logical = xcd * q + (xcd < rem ? xcd : rem) + bid / NXCD;
+        const int per_group = GROUP_M * tiles_n;
+        const int group = logical / per_group;
+        const int first_m = group * GROUP_M;
+        const int gsize = (tiles_m - first_m) < GROUP_M ? (tiles_m - first_m) : GROUP_M;
+        const int in_group = logical - group * per_group;
+        tm = first_m + in_group % gsize;
+        tn = in_group / gsize;
+    }
+    const int64_t m0 = int64_t(tm) * 128, n0 = int64_t(tn) * 128;
 
-    // direct loads: wave w writes panel rows [w*BM/4, (w+1)*BM/4) as 16-row
-    // chunks; lane L covers row L/4 of a chunk, LDS slot L%4 <- HBM chunk
-    // (L%4) ^ ((L/16)%4)
-    const int lrow = lane >> 2;
-    const int gchunk = (lane & 3) ^ ((lane >> 4) & 3);
-    const int64_t steps = int64_t(d + 1) * nk;
-    const int8_t* pa = SA + kstep0 * kbpa + (m0 + wave * (BM / 4) + lrow) * OZ_BK + 16 * gchunk;
-    const int8_t* pb = SB + int64_t(d) * spb + kstep0 * kbpb + (n0 + wave * (BN / 4) + lrow) * OZ_BK + 16 * gchunk;
-    int64_t kleft = nk;
+    // direct loads: one instruction of wave w writes rows 32w .. 32w+31 of a
+    // panel (1 KiB); lane L covers row 32w + L/2, LDS slot L%2 <- HBM chunk
+    // (L%2) ^ ((L/16)%2) (= the slot rule for that row)
+    const int lrow = 32 * wave + (lane >> 1);
+    const int gch = (lane & 1) ^ ((lane >> 4) & 1);
+    const int8_t* pa = SA + kstep0 * kbpa + (m0 + lrow) * OZ_KB + 16 * gch;
+    const int8_t* pb = SB + kstep0 * kbpb + (n0 + lrow) * OZ_KB + 16 * gch;
+    const int64_t steps = nk;
 
-    auto issue = [&](int stage) {
-        int8_t* la = smem + stage * STAGE + wave * (BM / 4) * OZ_BK;
-        int8_t* lb = smem + stage * STAGE + BM * OZ_BK + wave * (BN / 4) * OZ_BK;
+    auto issue = [&](int stage, int64_t st) {
+        const int64_t oa = st * kbpa;
+        const int64_t ob = st * kbpb;
+        int8_t* l = smem + stage * STAGE + wave * (32 * RB);
 #pragma unroll
-        for (int i = 0; i < LA; ++i)
-            __builtin_amdgcn_global_load_lds((gptr_t)(pa + 16 * i * OZ_BK), (lptr_t)(la + 16 * i * OZ_BK), 16, 0, 0);
+        for (int t = 0; t < T; ++t)
+            __builtin_amdgcn_global_load_lds((gptr_t)(pa + t * spa + oa), (lptr_t)(l + t * PANEL), 16, 0, 0);
 #pragma unroll
-        for (int i = 0; i < LB; ++i)
-            __builtin_amdgcn_global_load_lds((gptr_t)(pb + 16 * i * OZ_BK), (lptr_t)(lb + 16 * i * OZ_BK), 16, 0, 0);
-        pa += kbpa;
-        pb += kbpb;
-        if (--kleft == 0) {  // next slice pair (t+1, d-t-1)
-            kleft = nk;
-            pa += spa - nk * kbpa;
-            pb -= spb + nk * kbpb;
-        }
+        for (int u = 0; u < T; ++u)
+            __builtin_amdgcn_global_load_lds((gptr_t)(pb + u * spb + ob), (lptr_t)(l + (T + u) * PANEL), 16, 0, 0);
     };
 
-    // v_mfma_i32_32x32x32_i8: lane l supplies row (A) / column (B) l%32 and the
-    // 16 k-values of chunk l/32 of the 32-wide k half; result register v of
-    // lane l is row 8(v/4) + 4(l/32) + v%4, column l%32
-    mrm_i16 acc[PM][PN];
+    mrm_i16 acc[ND][2][2];
 #pragma unroll
-    for (int p = 0; p < PM; ++p)
+    for (int e = 0; e < ND; ++e)
 #pragma unroll
-        for (int c = 0; c < PN; ++c) acc[p][c] = mrm_i16{};
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[e][p][c] = mrm_i16{};
 
-    // Fragments of one 32-wide k half: lane l reads row frow of each block,
-    // 16-byte chunk 2h + l/32.
-    const int frow = lane & 31, fchunk = lane >> 5;
-    mrm_i4 fa[2][PM], fb[2][PN];
-    auto load_frags = [&](int stage, int h, mrm_i4* a, mrm_i4* b) {
-        const int8_t* As = smem + stage * STAGE;
-        const int8_t* Bs = As + BM * OZ_BK;
-#pragma unroll
-        for (int c = 0; c < PN; ++c) {
-            const int r = wn * (BN / 2) + 32 * c + frow;
-            b[c] = *reinterpret_cast<const mrm_i4*>(Bs + r * OZ_BK + 16 * oz_slot(r, 2 * h + fchunk));
-        }
-#pragma unroll
-        for (int p = 0; p < PM; ++p) {
-            const int r = wm * (BM / 2) + 32 * p + frow;
-            a[p] = *reinterpret_cast<const mrm_i4*>(As + r * OZ_BK + 16 * oz_slot(r, 2 * h + fchunk));
-        }
+    // fragment of block p (A) / c (B): lane l reads row l%32 of the block,
+    // 16-byte chunk l/32
+    const int frow = lane & 31, fch = lane >> 5;
+    auto frag = [&](const int8_t* P, int r) {
+        return *reinterpret_cast<const mrm_i4*>(P + r * RB + 16 * (fch ^ ((r >> 3) & 1)));
     };
-    auto mma = [&](const mrm_i4* a, const mrm_i4* b) {
-#pragma unroll
-        for (int p = 0; p < PM; ++p)
-#pragma unroll
-            for (int c = 0; c < PN; ++c) acc[p][c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[p], b[c], acc[p][c], 0, 0, 0);
-    };
-    // Wait until this thread's loads of step `ready` have landed, given that
-    // the loads of `later` further steps may stay in flight (an immediate
-    // count; fewer in flight near the end, then wait for all).
+    const int ra0 = wm * 64 + frow, rb0 = wn * 64 + frow;
+
+    // Wait until this thread's loads of the current step have landed while
+    // those of `later` further steps stay in flight (2T instructions a step).
+    static_assert(NSTAGE >= 2 && NSTAGE <= 3 && 2 * 2 * T < 64, "vmcnt range");
     auto wait_loads = [&](int64_t later) {
-        if (later >= S - 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((S - 1) * (LA + LB)) : "memory");
-        else if (later == S - 2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((S - 2) * (LA + LB)) : "memory");
+        if (NSTAGE >= 3 && later >= 2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * 2 * T) : "memory");
+        else if (later >= 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * T) : "memory");
         else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     };
-    // A plain s_barrier (__syncthreads() would also wait for the loads still
-    // in flight for later stages).
     auto barrier = [&]() {
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wave's LDS reads are done
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
     };
 
-    // Ring of S stages: step st lives in stage st % S.  Each K step computes
-    // its first half while the second half's fragments load, then waits for
-    // step st+1, refills the stage just consumed with step st+S and loads the
-    // next step's first fragments while the second half computes.
+    // The pairs are split by u into two halves of about equal size (half 0:
+    // u < UH).  A K step computes half 0 while half 1's fragments load, then
+    // waits for the next step, refills the stage just consumed and loads the
+    // next step's half-0 fragments while half 1 computes (the MFMAs keep
+    // running across the barrier).
+    constexpr int UH = oz_split_u(D0, ND);
+    mrm_i4 fa[2][T][2], fb[2][T][2];
+    auto load_half = [&](int stage, int h) {
+        const int8_t* As = smem + stage * STAGE;
+        const int8_t* Bs = As + T * PANEL;
 #pragma unroll
-    for (int i = 0; i < S; ++i)
-        if (i < steps) issue(i);
-    wait_loads(steps - 1 < S - 1 ? steps - 1 : S - 1);
+        for (int t = 0; t < T; ++t)
+            if (oz_a_used(D0, ND, UH, h, t)) {
+                fa[h][t][0] = frag(As + t * PANEL, ra0);
+                fa[h][t][1] = frag(As + t * PANEL, ra0 + 32);
+            }
+#pragma unroll
+        for (int u = 0; u < T; ++u)
+            if ((u < UH) == (h == 0)) {
+                fb[h][u][0] = frag(Bs + u * PANEL, rb0);
+                fb[h][u][1] = frag(Bs + u * PANEL, rb0 + 32);
+            }
+    };
+    auto mma_half = [&](int h) {
+#pragma unroll
+        for (int u = 0; u < T; ++u) {
+            if ((u < UH) != (h == 0)) continue;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                if (!oz_pair(D0, ND, t, u)) continue;
+                const int e = t + u - D0;
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                        acc[e][p][c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[h][t][p], fb[h][u][c], acc[e][p][c],
+                                                                             0, 0, 0);
+            }
+        }
+    };
+
+#pragma unroll
+    for (int i = 0; i < NSTAGE; ++i)
+        if (i < steps) issue(i, i);
+    wait_loads(steps - 1 < NSTAGE - 1 ? steps - 1 : NSTAGE - 1);
     barrier();
-    load_frags(0, 0, fa[0], fb[0]);
-    // (sched_barrier keeps the compiler from sinking the fragment reads
-    // behind the MFMAs they are meant to overlap; the lgkmcnt(0) before a
-    // batch of reads costs nothing -- the previous batch had a half step of
-    // MFMAs to land -- and keeps at most 8 reads in flight, which the
-    // compiler's own wait counting handles without falling back to a full wait)
+    load_half(0, 0);
     for (int64_t st = 0; st + 1 < steps; ++st) {
-        const int cur = int(st % S);
+        const int cur = int(st % NSTAGE);
         __builtin_amdgcn_s_waitcnt(0xC07F);
-        load_frags(cur, 1, fa[1], fb[1]);
+        load_half(cur, 1);
         __builtin_amdgcn_sched_barrier(0);
-        mma(fa[0], fb[0]);
-        // steps issued so far: up to min(steps-1, st+S-1)
-        const int64_t last = steps - 1 < st + S - 1 ? steps - 1 : st + S - 1;
+        mma_half(0);
+        // steps issued so far: up to min(steps-1, st+NSTAGE-1)
+        const int64_t last = steps - 1 < st + NSTAGE - 1 ? steps - 1 : st + NSTAGE - 1;
         wait_loads(last - (st + 1));
         barrier();
-        if (st + S < steps) issue(cur);
-        load_frags(int((st + 1) % S), 0, fa[0], fb[0]);
+        if (st + NSTAGE < steps) issue(cur, st + NSTAGE);
+        load_half(int((st + 1) % NSTAGE), 0);
         __builtin_amdgcn_sched_barrier(0);
-        mma(fa[1], fb[1]);
+        mma_half(1);
     }
     __builtin_amdgcn_s_waitcnt(0xC07F);
-    load_frags(int((steps - 1) % S), 1, fa[1], fb[1]);
-    mma(fa[0], fb[0]);
-    mma(fa[1], fb[1]);
+    load_half(int((steps - 1) % NSTAGE), 1);
+    mma_half(0);
+    mma_half(1);
 
-    const int scale = -7 * (d + 2);
+    // C (op)= diagonal D0, then (+/-)= D0+1, ...: the roundings of one launch
+    // per diagonal.  Result register v of lane l is row 8(v/4) + 4(l/32) + v%4,
+    // column l%32 of its block.
+    const int later = op == MRM_DO_SUB ? MRM_DO_SUB : MRM_DO_ADD;
 #pragma unroll
-    for (int p = 0; p < PM; ++p) {
+    for (int p = 0; p < 2; ++p) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const int64_t row = m0 + wm * (BM / 2) + 32 * p + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            const int64_t row = m0 + wm * 64 + 32 * p + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
             if (row >= M) continue;
-            const int er = oz_exponent(ea[row]) + scale;
+            const int er = oz_exponent(ea[row]);
             double* crow = C + row * ldc;
 #pragma unroll
-            for (int c = 0; c < PN; ++c) {
-                const int64_t col = n0 + wn * (BN / 2) + 32 * c + (lane & 31);
-                if (col < N) {
-                    const double x = ldexp(double(acc[p][c][v]), er + oz_exponent(eb[col]));
-                    if (op == MRM_DO_NONE) crow[col] = x;
-                    else if (op == MRM_DO_ADD) crow[col] = crow[col] + x;
-                    else crow[col] = crow[col] - x;
+            for (int c = 0; c < 2; ++c) {
+                const int64_t col = n0 + wn * 64 + 32 * c + (lane & 31);
+                if (col >= N) continue;
+                const int ec = er + oz_exponent(eb[col]);
+                double x = op == MRM_DO_NONE ? 0.0 : crow[col];
+#pragma unroll
+                for (int e = 0; e < ND; ++e) {
+                    const double y = ldexp(double(acc[e][p][c][v]), ec - 7 * (D0 + e + 2));
+                    const int o = e == 0 ? op : later;
+                    x = o == MRM_DO_NONE ? y : (o == MRM_DO_ADD ? x + y : x - y);
                 }
+                crow[col] = x;
             }
         }
     }
 }
 
-// 128 x 256 workgroup tiles, three stages (72 KiB of LDS), two workgroups per
-// CU: measured 10% faster than 256 x 256 with four stages and one workgroup
-// per CU, the extra occupancy hiding more of the L2 latency.
-constexpr int OZ_PM = 2, OZ_PN = 4, OZ_STAGES = 3;
-constexpr int OZ_TILE = 256;  // slices are padded to whole tiles of either side
-static_assert(64 * OZ_PM <= OZ_TILE && 64 * OZ_PN <= OZ_TILE && OZ_TILE % (64 * OZ_PM) == 0, "tile padding");
+// Three LDS stages while they fit in the 160 KiB of a CU, else two.
+constexpr int oz_fstages(int T) { return 3 * 2 * T * 128 * 32 <= 160 * 1024 ? 3 : 2; }
+
+template <int D0, int ND>
+const void* oz_fused_fn() {
+    return reinterpret_cast<const void*>(&oz_fused_kernel<D0, ND, oz_fstages(D0 + ND)>);
+}
+
+// Diagonal groups per slice count, one launch each.  A group of ND diagonals
+// holds ND int32 accumulator sets of 64 registers per lane, so ND <= 4 (five
+// and six spill); within that, larger groups load fewer slice panels per pair.
+// Measured at 8192^3 per group (ms): (0,4) 5.26 against (0,3)+(3,1) 6.65 and
+// (0,2)+(2,2) 6.37; s = 5: (0,2)+(2,3) 8.38 against (0,3)+(3,2) 8.78; s = 6:
+// (0,2)+(2,4) 11.11 against (0,3)+(3,3) 11.57 and (0,4)+(4,2) 11.94; s = 7:
+// (0,3)+(3,4) 14.18 against (0,4)+(4,3) 14.52.
+struct OzGroup {
+    int d0, nd;
+};
+constexpr int OZ_MAX_GROUPS = 3;
+constexpr OzGroup OZ_GROUPS[MRM_OZAKI_MAX_SLICES + 1][OZ_MAX_GROUPS] = {
+    {},
+    {{0, 1}},
+    {{0, 2}},
+    {{0, 3}},
+    {{0, 4}},
+    {{0, 2}, {2, 3}},
+    {{0, 2}, {2, 4}},
+    {{0, 3}, {3, 4}},
+    {{0, 3}, {3, 4}, {7, 1}},
+};
+static_assert(MRM_OZAKI_MAX_SLICES == 8, "a group table row per slice count");
+
+const void* oz_fused_for(int d0, int nd) {
+    switch (d0 * 8 + nd) {
+        case 0 * 8 + 1: return oz_fused_fn<0, 1>();
+        case 0 * 8 + 2: return oz_fused_fn<0, 2>();
+        case 0 * 8 + 3: return oz_fused_fn<0, 3>();
+        case 0 * 8 + 4: return oz_fused_fn<0, 4>();
+        case 2 * 8 + 3: return oz_fused_fn<2, 3>();
+        case 2 * 8 + 4: return oz_fused_fn<2, 4>();
+        case 3 * 8 + 4: return oz_fused_fn<3, 4>();
+        case 7 * 8 + 1: return oz_fused_fn<7, 1>();
+        default: return nullptr;
+    }
+}
+
+constexpr int OZ_TILE = 128;  // slices are padded to whole tiles of either side
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
@@ -352,7 +431,7 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
         mrm_set_error(MRM_BAD_DIMS, "ozaki: M or N of 2^22 or more, or K of 2^24 or more (launch grid limits)");
         return MRM_BAD_DIMS;
     }
-    const int64_t Mp = round_up(M, OZ_TILE), Np = round_up(N, OZ_TILE), Kp = round_up(K, OZ_BK);
+    const int64_t Mp = round_up(M, OZ_TILE), Np = round_up(N, OZ_TILE), Kp = round_up(K, OZ_KPAD);
     const size_t need = mrm_ozaki_workspace(M, N, K, slices);
     if (!work || work_bytes < need) {
         mrm_set_error(MRM_BAD_ARG, "ozaki: workspace of %zu bytes needed, %zu given", need, work_bytes);
@@ -383,19 +462,36 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
                        M, K, ea, SA, Mp, spa, slices);
     hipLaunchKernelGGL(oz_slice_kernel, dim3(unsigned(Kp / 64), unsigned(Np / 64)), dim3(256), 0, stream, B, brs, bks,
                        N, K, eb, SB, Np, spb, slices);
-    // per K chunk, diagonals largest first; the very first launch applies op,
-    // every later one accumulates with the sign op asks for
-    const int tiles_m = int(Mp / (64 * OZ_PM)), tiles_n = int(Np / (64 * OZ_PN));
-    const int64_t ksteps = Kp / OZ_BK, chunk = MRM_OZAKI_KCHUNK / OZ_BK;
+    // per K chunk, diagonals largest first in the groups of OZ_GROUPS[slices];
+    // the very first diagonal applies op, every later one accumulates with
+    // the sign op asks for
+    const int tiles_m = int(Mp / 128), tiles_n = int(Np / 128);
+    const int64_t ksteps = Kp / OZ_KB, chunk = MRM_OZAKI_KCHUNK / OZ_KB;
+    int64_t kbpa = Mp * OZ_KB, kbpb = Np * OZ_KB;
     bool first = true;
     for (int64_t k0 = 0; k0 < ksteps; k0 += chunk) {
-        const int64_t nk = ksteps - k0 < chunk ? ksteps - k0 : chunk;
-        for (int d = 0; d < slices; ++d) {
-            const int o = first ? op : (op == MRM_DO_SUB ? MRM_DO_SUB : MRM_DO_ADD);
+        int64_t nk = ksteps - k0 < chunk ? ksteps - k0 : chunk;
+        int64_t kk = k0;
+        for (int g = 0; g < OZ_MAX_GROUPS && OZ_GROUPS[slices][g].nd > 0; ++g) {
+            const int d0 = OZ_GROUPS[slices][g].d0, nd = OZ_GROUPS[slices][g].nd;
+            int o = first ? op : (op == MRM_DO_SUB ? MRM_DO_SUB : MRM_DO_ADD);
             first = false;
-            hipLaunchKernelGGL((oz_gemm_kernel<OZ_PM, OZ_PN, OZ_STAGES>), dim3(unsigned(tiles_m) * unsigned(tiles_n)),
-                               dim3(OZ_THREADS), 0, stream, SA, SB, Mp * OZ_BK, Np * OZ_BK, spa, spb, k0, nk, d, ea, eb, C, ec, M, N, o,
-                               tiles_m, tiles_n);
+            const void* fn = oz_fused_for(d0, nd);
+            if (!fn) {
+                mrm_set_error(MRM_BAD_ARG, "ozaki: no kernel for diagonals %d..%d", d0, d0 + nd - 1);
+                return MRM_BAD_ARG;
+            }
+            int64_t Mv = M, Nv = N, ecv = ec;
+            int tmv = tiles_m, tnv = tiles_n;
+            const int* eav = ea;
+            const int* ebv = eb;
+            const int8_t* sav = SA;
+            const int8_t* sbv = SB;
+            int64_t spav = spa, spbv = spb;
+            double* cv = C;
+            void* args[] = {&sav, &sbv, &kbpa, &kbpb, &spav, &spbv, &kk, &nk, &eav, &ebv, &cv, &ecv, &Mv, &Nv, &o, &tmv, &tnv};
+            MRM_HIP_TRY(hipLaunchKernel(fn, dim3(unsigned(tiles_m) * unsigned(tiles_n)), dim3(OZ_THREADS), args, 0,
+                                        stream));
         }
     }
     MRM_HIP_TRY(hipGetLastError());
@@ -403,6 +499,6 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
 }
 
 size_t mrm_ozaki_workspace(int64_t M, int64_t N, int64_t K, int slices) {
-    const int64_t Mp = round_up(M, OZ_TILE), Np = round_up(N, OZ_TILE), Kp = round_up(K, OZ_BK);
+    const int64_t Mp = round_up(M, OZ_TILE), Np = round_up(N, OZ_TILE), Kp = round_up(K, OZ_KPAD);
     return size_t(round_up((Mp + Np) * 4, 256)) + size_t(slices) * size_t(Mp + Np) * size_t(Kp);
 }

@@ -40,6 +40,22 @@ def test_bit_exact_vs_oracle(trans, M, N, K, s):
     assert np.array_equal(Cg, Co)
 
 
+@pytest.mark.parametrize("s", range(1, 9))
+@pytest.mark.parametrize("op", [ss.doNone, ss.doSub])
+def test_every_slice_count_bit_exact(s, op):
+    """Every slice count runs its own launches (groups of up to four slice
+    diagonals per launch); each must equal the oracle, with none and with the
+    signed accumulation of doSub across the groups."""
+    M, N, K = 200, 150, 300
+    A = orc.pm1_wide_matrix(K, M, 300 + s)
+    B = orc.pm1_wide_matrix(N, K, 400 + s)
+    C0 = orc.pm1_matrix(N, M, 500 + s)
+    Cg = C0.copy()
+    ss.MatrixMultOzaki(Cg, A, B, slices=s, op=op)
+    Co = orc.ozaki_gemm(A, B, s, C0.copy(), op=op)
+    assert np.array_equal(Cg, Co)
+
+
 @pytest.mark.parametrize("op", [ss.doAdd, ss.doSub])
 def test_ops_bit_exact(op):
     M, N, K = 70, 90, 130

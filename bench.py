@@ -315,14 +315,17 @@ def ozaki_row(lib, reps: int = 3):
 def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
     """A X = B with A = U diag(s) V^T (U, V Householder-QR factors of Gaussian
     matrices, s log-spaced to kappa), B Gaussian: plain FP64 LU + solve against
-    LU with FP32 / BF16 trailing updates and FP64 refinement, all to a normwise
-    backward error <= 1e-14.  Times are wall clock around the synchronous
+    LU with FP32 / BF16 trailing updates and FP64 refinement (plain steps, then
+    GMRES corrections preconditioned by the factors if those stall), all to a
+    normwise backward error <= 1e-14.  Times are wall clock around the synchronous
     device entry (factorisation, solves, residuals, norms): one untimed call
     per mode first (it grows the library's work buffers), then the median of
     3 calls.  The backward error is also recomputed from an independent
     residual (torch's FP64 matmul)."""
     import ctypes as C
     import torch
+
+    from mrmath_amd import _lib
 
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev).manual_seed(4401)
@@ -338,13 +341,13 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
         res = {}
         for mode, name in ((0, "fp64"), (1, "fp32_ir"), (2, "bf16_ir")):
             X = torch.zeros(n, nrhs, dtype=torch.float64, device=dev)
-            it, be, fb = C.c_int32(), C.c_double(), C.c_int32()
+            st = _lib.SolveStats()
             ts = []
             for rep in range(4):
                 torch.cuda.synchronize()
                 t = time.perf_counter()
-                rc = lib.mrm_dev_lineq_solve_ir(mode, A.data_ptr(), n * 8, n, B.data_ptr(), nrhs * 8, X.data_ptr(),
-                                                nrhs * 8, nrhs, 50, 1e-14, C.byref(it), C.byref(be), C.byref(fb), sp)
+                rc = lib.mrm_dev_lineq_solve_ir_stats(mode, A.data_ptr(), n * 8, n, B.data_ptr(), nrhs * 8,
+                                                      X.data_ptr(), nrhs * 8, nrhs, 50, 1e-14, C.byref(st), sp)
                 torch.cuda.synchronize()
                 if rep > 0:
                     ts.append(time.perf_counter() - t)
@@ -352,9 +355,14 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
             berr = float((R.abs().amax(dim=0) / (A.abs().sum(dim=1).amax() * X.abs().amax(dim=0) +
                                                  B.abs().amax(dim=0))).max())
             del R
-            res[name] = {"rc": rc, "ms": round(1e3 * statistics.median(ts), 1), "refinement_steps": it.value,
-                         "backward_error": be.value, "backward_error_independent": berr,
-                         "fell_back_to_fp64": bool(fb.value)}
+            res[name] = {"rc": rc, "ms": round(1e3 * statistics.median(ts), 1), "refinement_steps": st.iterations,
+                         "gmres_ir": bool(st.gmres), "krylov_steps": st.krylov_steps,
+                         "backward_error": st.backward_error, "backward_error_independent": berr,
+                         "fell_back_to_fp64": bool(st.fell_back)}
+            if st.fell_back:
+                res[name]["low_precision_attempt"] = {"steps": st.lowp_iterations,
+                                                      "krylov_steps": st.lowp_krylov_steps,
+                                                      "backward_error": st.lowp_backward_error}
         for name in ("fp32_ir", "bf16_ir"):
             res[name]["speedup_vs_fp64"] = round(res["fp64"]["ms"] / res[name]["ms"], 3)
         out["kappa"][f"{kappa:g}"] = res

@@ -266,14 +266,39 @@ MRM_API int32_t mrm_dev_gemm_lowp(int32_t mode, int64_t M, int64_t N, int64_t K,
  * on the FP32 / BF16 matrix cores, FP64 panels and pivoting), then FP64
  * refinement x += (LU)^-1 (b - A x) runs until the normwise backward error
  * max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) <= tol or
- * max_iter steps.  A low-precision run whose refinement stalls is redone in
- * FP64 (*fell_back = 1).  Returns MRM_OK, MRM_SINGULAR or MRM_NOT_CONVERGED
+ * max_iter steps.  A low-precision run whose plain refinement stalls continues
+ * with GMRES corrections (see mrm_dev_lineq_solve_ir_stats); one whose GMRES
+ * refinement stalls too is redone in FP64 (*fell_back = 1).  Returns MRM_OK, MRM_SINGULAR or MRM_NOT_CONVERGED
  * (X then holds the last iterate).  iterations / backward_error / fell_back
  * may be NULL.  Synchronises `stream`. */
 MRM_API int32_t mrm_dev_lineq_solve_ir(int32_t mode, const double* A, int64_t lda, int64_t n, const double* B,
                                        int64_t ldb, double* X, int64_t ldx, int64_t nrhs, int32_t max_iter,
                                        double tol, int32_t* iterations, double* backward_error, int32_t* fell_back,
                                        void* stream);
+
+/* What a refined solve did (mrm_dev_lineq_solve_ir_stats). */
+typedef struct mrm_solve_stats {
+    int32_t iterations;         /* refinement steps of the returned solution */
+    int32_t fell_back;          /* 1: the low-precision run stalled, the LU was redone in FP64 */
+    int32_t gmres;              /* 1: the plain step stalled and GMRES preconditioned by the
+                                   factors supplied the corrections (GMRES-IR) */
+    int32_t krylov_steps;       /* Krylov steps over all GMRES corrections of the solution */
+    double backward_error;      /* normwise backward error of the returned solution */
+    int32_t lowp_iterations;    /* after a fall-back: steps of the low-precision attempt */
+    int32_t lowp_krylov_steps;  /* and its Krylov steps */
+    double lowp_backward_error; /* and its last backward error */
+} mrm_solve_stats;
+
+/* mrm_dev_lineq_solve_ir with every statistic of the run (stats may be NULL).
+ * With FP32 / BF16 factors, when plain refinement stops halving the backward
+ * error the solve continues with GMRES corrections preconditioned by the same
+ * factors (one Krylov space per right-hand side, FP64 products by A, classical
+ * Gram-Schmidt twice; Carson and Higham's GMRES-IR) before falling back to an
+ * FP64 factorisation.  The reference refines with the plain step only
+ * (src/LinAlgLU.pas:542-586). */
+MRM_API int32_t mrm_dev_lineq_solve_ir_stats(int32_t mode, const double* A, int64_t lda, int64_t n, const double* B,
+                                             int64_t ldb, double* X, int64_t ldx, int64_t nrhs, int32_t max_iter,
+                                             double tol, mrm_solve_stats* stats, void* stream);
 
 /* MatrixLinEQSolve (src/LinAlgLU.pas:481-592) with a factorisation precision.
  * mode MRM_SOLVE_FP64 is mrm_lineq_solve; the IR modes refine to a backward

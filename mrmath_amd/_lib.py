@@ -27,6 +27,13 @@ class Stats(C.Structure):
                 ("device_gemms", C.c_int64)]
 
 
+class SolveStats(C.Structure):
+    """mrm_solve_stats: what a refined solve did."""
+    _fields_ = [("iterations", C.c_int32), ("fell_back", C.c_int32), ("gmres", C.c_int32),
+                ("krylov_steps", C.c_int32), ("backward_error", C.c_double), ("lowp_iterations", C.c_int32),
+                ("lowp_krylov_steps", C.c_int32), ("lowp_backward_error", C.c_double)]
+
+
 _thr = [_pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i32]
 
 # name -> (restype, argtypes)
@@ -60,6 +67,8 @@ SIGNATURES = {
     "mrm_dev_gemm_lowp": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _pv, _i64, _pv]),
     "mrm_dev_lineq_solve_ir": (_i32, [_i32, _pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i32, C.c_double, _pv, _pv,
                                       _pv, _pv]),
+    "mrm_dev_lineq_solve_ir_stats": (_i32, [_i32, _pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i32, C.c_double,
+                                            C.POINTER(SolveStats), _pv]),
     "mrm_lineq_solve_ex": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64, _i32]),
     "mrm_batched_dgemm_plan": (_i32, [_pv, _i64, _i32, _pv]),
     "mrm_batched_lu_plan": (_i32, [_pv, _i64, _i32, _pv, _pv]),

@@ -125,6 +125,7 @@ void release_all() {
         }
         mrm::lu_release_workspace();
         mrm::ir_release_workspace();
+        mrm::gmres_release_workspace();
     }
     (void)hipStreamDestroy(g_ctx.stream);
     g_ctx.stream = nullptr;

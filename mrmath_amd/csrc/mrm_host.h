@@ -20,4 +20,5 @@ namespace mrm {
 // hold mrm_host_mutex() and no work using them is in flight.
 void lu_release_workspace();
 void ir_release_workspace();
+void gmres_release_workspace();
 }  // namespace mrm

@@ -9,8 +9,10 @@
 //   max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
 // reaches the tolerance, every residual formed in FP64 from the original A:
 //   x = (LU)^-1 P b;  repeat r = b - A x;  x += (LU)^-1 P r.
-// Refinement that stops shrinking the backward error (a factorisation too
-// inaccurate for the conditioning) falls back to the FP64 factorisation.
+// When the plain step stops halving the backward error (kappa(A) u_f near or
+// above 1 for the factors' precision u_f), the corrections d ~= A^-1 r come
+// from GMRES preconditioned by the same factors instead (GMRES-IR,
+// mrm_gmres.hip); if that stalls too, the factorisation is redone in FP64.
 
 #include <cmath>
 #include <cstdint>
@@ -18,11 +20,18 @@
 #include <vector>
 
 #include "mrm_common.h"
+#include "mrm_gmres.h"
 #include "mrm_host.h"
 #include "mrm_ir.h"
 #include "mrm_lu.h"
 
 namespace {
+
+// A plain refinement step of low-precision factors must shrink the backward
+// error at least this much, or the corrections switch to GMRES: each GMRES
+// Krylov step costs one solve, like a plain step, but contracts much faster
+// when kappa(A) u_f is near 1 (plain steps then gain less than a factor 10).
+constexpr double IR_PLAIN_RATE = 0.1;
 
 // The norms are combined with atomicMax on the bit patterns of non-negative
 // doubles, which order like the values; a NaN is recorded as the quiet NaN
@@ -138,6 +147,7 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
     hipLaunchKernelGGL(colmax_kernel, dim3(rowblocks), dim3(256), 0, s, B, ldb, int64_t(n), int64_t(nrhs), bn);
     std::vector<unsigned long long> h(size_t(1 + 3 * nrhs));
     double prev = INFINITY;
+    bool gmres = false;  // corrections from GMRES preconditioned by the factors
     for (int it = 0;; ++it) {
         // r = b - A x in FP64
         MRM_HIP_TRY(hipMemcpy2DAsync(w.r, size_t(ldr) * 8, B, size_t(ldb) * 8, size_t(nrhs) * 8, size_t(n),
@@ -165,9 +175,29 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
         st.iterations = it;
         st.backward_error = berr;
         if (berr <= tol) return MRM_OK;
-        if (it >= max_iter || !(berr < 0.5 * prev)) return MRM_NOT_CONVERGED;
+        if (it >= max_iter || std::isnan(berr)) return MRM_NOT_CONVERGED;
+        if (gmres ? !(berr < 0.5 * prev) : !(berr < IR_PLAIN_RATE * prev)) {
+            // the plain step contracts slowly or not at all: with
+            // low-precision factors continue with GMRES corrections
+            // preconditioned by them; a GMRES step that does not halve the
+            // backward error (or any stall in FP64) ends the run
+            if (gmres || mode == MRM_SOLVE_FP64) {
+                if (!(berr < 0.5 * prev)) return MRM_NOT_CONVERGED;
+            } else {
+                gmres = true;
+                st.gmres = 1;
+            }
+        }
         prev = berr;
-        if ((rc = mrm::lu_solve<double>(w.lu, n, n, w.perm, w.r, ldr, w.d, ldr, nrhs, s, dinv))) return rc;
+        if (gmres) {
+            int k = 0;
+            rc = mrm::gmres_correction(A, lda, n, w.lu, w.perm, dinv, w.r, ldr, w.d, nrhs, mrm::IR_GMRES_DIM,
+                                       mrm::IR_GMRES_TOL, &k, s);
+            st.krylov_steps += k;
+        } else {
+            rc = mrm::lu_solve<double>(w.lu, n, n, w.perm, w.r, ldr, w.d, ldr, nrhs, s, dinv);
+        }
+        if (rc) return rc;
         mrm::add_inplace<double>(X, ldx, w.d, ldr, n, nrhs, s);
     }
 }
@@ -215,9 +245,11 @@ int lineq_solve_ir(int mode, const double* A, int64_t lda, int n, const double* 
         // conditioning (or hit an exactly zero pivot the FP64 updates might
         // not): redo the factorisation in FP64
         const IrStats first = st;
+        st = IrStats();
         rc = factor_solve_refine(MRM_SOLVE_FP64, A, lda, n, B, ldb, X, ldx, nrhs, max_iter, tol, st, s);
         st.lowp_iterations = first.iterations;
         st.lowp_backward_error = first.backward_error;
+        st.lowp_krylov_steps = first.krylov_steps;
         st.fell_back = 1;
     }
     if (stats) *stats = st;

@@ -404,6 +404,18 @@ int32_t mrm_dev_gemm_lowp(int32_t mode, int64_t M, int64_t N, int64_t K, const d
 int32_t mrm_dev_lineq_solve_ir(int32_t mode, const double* A, int64_t lda, int64_t n, const double* B, int64_t ldb,
                                double* X, int64_t ldx, int64_t nrhs, int32_t max_iter, double tol,
                                int32_t* iterations, double* backward_error, int32_t* fell_back, void* stream) {
+    mrm_solve_stats st;
+    const int32_t rc = mrm_dev_lineq_solve_ir_stats(mode, A, lda, n, B, ldb, X, ldx, nrhs, max_iter, tol, &st, stream);
+    if (iterations) *iterations = st.iterations;
+    if (backward_error) *backward_error = st.backward_error;
+    if (fell_back) *fell_back = st.fell_back;
+    return rc;
+}
+
+int32_t mrm_dev_lineq_solve_ir_stats(int32_t mode, const double* A, int64_t lda, int64_t n, const double* B,
+                                     int64_t ldb, double* X, int64_t ldx, int64_t nrhs, int32_t max_iter, double tol,
+                                     mrm_solve_stats* stats, void* stream) {
+    if (stats) *stats = mrm_solve_stats{};
     int rc = mrm_ensure_init();
     if (rc) return rc;
     if ((rc = check_square(n, lda))) return rc;
@@ -420,9 +432,16 @@ int32_t mrm_dev_lineq_solve_ir(int32_t mode, const double* A, int64_t lda, int64
     mrm::IrStats st;
     rc = mrm::lineq_solve_ir(mode, A, lda / 8, int(n), B, ldb / 8, X, ldx / 8, int(nrhs), max_iter, tol, &st,
                              mrm_resolve_stream(stream));
-    if (iterations) *iterations = st.iterations;
-    if (backward_error) *backward_error = st.backward_error;
-    if (fell_back) *fell_back = st.fell_back;
+    if (stats) {
+        stats->iterations = st.iterations;
+        stats->fell_back = st.fell_back;
+        stats->gmres = st.gmres;
+        stats->krylov_steps = st.krylov_steps;
+        stats->backward_error = st.backward_error;
+        stats->lowp_iterations = st.lowp_iterations;
+        stats->lowp_krylov_steps = st.lowp_krylov_steps;
+        stats->lowp_backward_error = st.lowp_backward_error;
+    }
     return rc;
 }
 

@@ -126,13 +126,58 @@ def test_lu_ir_pivots_from_many_panel_workgroups(n, boost_from):
     assert be <= 1e-14 and orc.backward_error(A, X, B) <= 1e-14
 
 
+def _solve_ir_stats(mode, A, B, max_iter=50, tol=1e-14):
+    import torch
+    lib = _lib.load()
+    n, nrhs = B.shape
+    dA, dB = _dev(A), _dev(B)
+    dX = torch.zeros(n, nrhs, dtype=torch.float64, device="cuda")
+    st = _lib.SolveStats()
+    rc = lib.mrm_dev_lineq_solve_ir_stats(mode, dA.data_ptr(), n * 8, n, dB.data_ptr(), nrhs * 8, dX.data_ptr(),
+                                          nrhs * 8, nrhs, max_iter, tol, C.byref(st),
+                                          torch.cuda.current_stream().cuda_stream)
+    return rc, dX.cpu().numpy(), st
+
+
+@pytest.mark.parametrize("n,kappa,nrhs", [(1000, 1e2, 4), (2500, 1e2, 64)])
+def test_gmres_ir_keeps_bf16_factors(n, kappa, nrhs):
+    """BF16 trailing updates leave factors too inexact for plain refinement
+    (kappa u_bf16 near 1 at these sizes): the plain step stalls, GMRES
+    preconditioned by the same factors supplies the corrections, and the
+    solve reaches a 1e-14 backward error without the FP64 fallback."""
+    A, B = _system(n, kappa, nrhs, int(n + np.log10(kappa)))
+    rc, X, st = _solve_ir_stats(_lib.SOLVE_BF16_IR, A, B)
+    print(f"n={n} kappa={kappa:g}: rc={rc} steps={st.iterations} gmres={st.gmres} krylov={st.krylov_steps} "
+          f"berr={st.backward_error:.3e} fell_back={st.fell_back}")
+    assert rc == 0 and st.fell_back == 0
+    assert st.gmres == 1 and st.krylov_steps > 0
+    be = orc.backward_error(A, X, B)
+    assert st.backward_error <= 1e-14 and be <= 1e-14, (st.backward_error, be)
+    rc, Xo = orc.lineq_solve(A, B, 0)
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= kappa * 1e-13
+
+
+def test_gmres_ir_stats_without_a_stall():
+    """A well-conditioned system with FP32 factors converges by plain
+    refinement: no GMRES step, no fallback, and the statistics entry agrees
+    with the three-output entry."""
+    A, B = _system(600, 1e2, 3, 5)
+    rc, X, st = _solve_ir_stats(_lib.SOLVE_FP32_IR, A, B)
+    assert rc == 0 and st.gmres == 0 and st.krylov_steps == 0 and st.fell_back == 0
+    rc2, X2, it, be, fb = _solve_ir(_lib.SOLVE_FP32_IR, A, B)
+    assert rc2 == 0 and it == st.iterations and fb == 0
+    assert np.array_equal(X, X2) and be == st.backward_error
+
+
 def test_lu_ir_ill_conditioned_falls_back_or_converges():
     """kappa = 1e10 is beyond what FP32 factors can refine; the solve must still
     return a 1e-14 backward-error solution, through the FP64 fallback if needed."""
     A, B = _system(800, 1e10, 4, 99)
     for mode in (_lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR):
-        rc, X, it, be, fb = _solve_ir(mode, A, B)
-        print(f"mode={mode} kappa=1e10: rc={rc} steps={it} berr={be:.3e} fell_back={fb}")
+        rc, X, st = _solve_ir_stats(mode, A, B)
+        print(f"mode={mode} kappa=1e10: rc={rc} steps={st.iterations} gmres={st.gmres} krylov={st.krylov_steps} "
+              f"berr={st.backward_error:.3e} fell_back={st.fell_back} (low-precision attempt: "
+              f"{st.lowp_iterations} steps, {st.lowp_krylov_steps} Krylov, berr {st.lowp_backward_error:.3e})")
         assert rc == 0
         assert orc.backward_error(A, X, B) <= 1e-14
 
@@ -203,7 +248,7 @@ def system_16384():
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR])
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR])
 def test_n16384_64rhs_independent_residual(mode, system_16384):
     """The LU-IR workload itself (n = 16384, 64 right-hand sides, kappa = 1e2):
     the backward error is recomputed here from an independent FP64 residual

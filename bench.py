@@ -371,9 +371,11 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
 
 
 def batched_row(lib, reps: int = 5):
-    """8192 FP64 GEMMs with m, n, k uniform in [16, 256] (seed 5101) and 20000
-    solves with n uniform in [8, 128], A Gaussian + n I, one right-hand side
-    (seed 5201), each as one hipGraph replay of its plan (median of `reps`)."""
+    """8192 GEMMs with m, n, k uniform in [16, 256] (seed 5101; FP64, and FP32
+    pieces with error compensation) and 20000 solves with n uniform in
+    [8, 128], A Gaussian + n I, one right-hand side (seed 5201; FP64, and FP32
+    factors with FP64 refinement), each as one hipGraph replay of its plan
+    (median of `reps`)."""
     import torch
     from mrmath_amd import batched as bt
 
@@ -409,7 +411,17 @@ def batched_row(lib, reps: int = 5):
                          "gbps": round(nbytes / ms / 1e6, 1), "roofline_floor_ms": round(floor_ms, 4),
                          "frac_of_floor": round(floor_ms / ms, 4)}}
     gp.close()
-    del probs
+    # FP32 pieces with error compensation, FP64 chunk sums (the FP32 mode):
+    # time, and the normwise error of a sample against the FP64 plan's result
+    ref = [p[2].clone() for p in probs[:64]]
+    gp = bt.dgemm_plan(probs, 0, 1)
+    ms = timed(gp)
+    err = max(float(torch.linalg.norm(p[2] - r) / (torch.linalg.norm(p[0]) * torch.linalg.norm(p[1])))
+              for p, r in zip(probs[:64], ref))
+    out["gemm_8192_fp32_compensated"] = {"ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 2), "tile": gp.tile,
+                                         "normwise_err_vs_fp64_sample": err}
+    gp.close()
+    del probs, ref
     rng = np.random.default_rng(5201)
     ns = [int(v) for v in rng.integers(8, 129, 20000)]
     g = torch.Generator(device=dev).manual_seed(5201)

@@ -360,6 +360,15 @@ typedef struct mrm_lu_desc {
  * small-size path FMAMatrixMult, src/FMAMatrixOperations.pas:94-143). */
 MRM_API int32_t mrm_batched_dgemm_plan(const mrm_gemm_desc* problems, int64_t count, int32_t tile, void** plan);
 
+/* mrm_batched_dgemm_plan with a precision: MRM_SOLVE_FP64, or
+ * MRM_SOLVE_FP32_IR -- every operand split into FP32 pieces x = xh + xl, the
+ * product Ah Bh + (Ah Bl + Al Bh) formed on the FP32 matrix cores (error
+ * compensation after Ootomo and Yokota) with the sums of each K chunk of 16
+ * added in FP64: an error of FP32 sums over 16 products, not an FP64 result
+ * (tiles 64 / 128 or 0).  The reference multiplies in FP64 only. */
+MRM_API int32_t mrm_batched_dgemm_plan_ex(const mrm_gemm_desc* problems, int64_t count, int32_t tile, int32_t mode,
+                                          void** plan);
+
 /* Plans `count` linear solves A_i x_i = b_i (n_i <= 128, one right-hand side)
  * with partial pivoting, grouped by size class into persistent launches
  * captured in a hipGraph; mode MRM_SOLVE_FP64, or MRM_SOLVE_FP32_IR (FP32

@@ -71,6 +71,7 @@ SIGNATURES = {
                                             C.POINTER(SolveStats), _pv]),
     "mrm_lineq_solve_ex": (_i32, [_pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i64, _i32]),
     "mrm_batched_dgemm_plan": (_i32, [_pv, _i64, _i32, _pv]),
+    "mrm_batched_dgemm_plan_ex": (_i32, [_pv, _i64, _i32, _i32, _pv]),
     "mrm_batched_lu_plan": (_i32, [_pv, _i64, _i32, _pv, _pv]),
     "mrm_batched_run": (_i32, [_pv, _pv]),
     "mrm_batched_plan_tile": (_i32, [_pv]),

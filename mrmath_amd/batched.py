@@ -58,15 +58,17 @@ class _Plan:
             pass
 
 
-def dgemm_plan(problems, tile: int = 0) -> _Plan:
+def dgemm_plan(problems, tile: int = 0, mode: int = _lib.SOLVE_FP64) -> _Plan:
     """problems: sequence of (A, B, C) float64 device tensors, C = A @ B.
-    tile 0 lets the autotuner pick 32, 64 or 128 (runs the batch)."""
+    tile 0 lets the autotuner pick 32, 64 or 128 (runs the batch).  mode
+    SOLVE_FP32_IR: FP32 pieces with error compensation on the FP32 matrix
+    cores, FP64 sums per K chunk of 16 (tiles 64 / 128)."""
     descs = (GemmDesc * max(len(problems), 1))()
     for i, (A, B, Cm) in enumerate(problems):
         descs[i] = GemmDesc(A.shape[0], B.shape[1], A.shape[1], A.data_ptr(), _rowstride(A), B.data_ptr(),
                             _rowstride(B), Cm.data_ptr(), _rowstride(Cm))
     h = C.c_void_p()
-    _lib.check(_lib.load().mrm_batched_dgemm_plan(descs, len(problems), tile, C.byref(h)))
+    _lib.check(_lib.load().mrm_batched_dgemm_plan_ex(descs, len(problems), tile, mode, C.byref(h)))
     return _Plan(h)
 
 

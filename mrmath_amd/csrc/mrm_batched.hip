@@ -10,6 +10,8 @@
 //     problems form one list, and a persistent grid of workgroups takes tiles
 //     from an atomic counter (largest K first).  T in {32, 64, 128} is picked
 //     by timing the actual batch (the autotuner) unless the caller fixes it.
+//     FP64 on the FP64 matrix cores, or FP32 pieces with error compensation
+//     on the FP32 matrix cores and FP64 accumulation per K chunk.
 //   LU solve: systems are grouped by size class (n <= 32, 64, 96, 128); per class
 //     one persistent launch where each wave (up to 64 unknowns) or each
 //     workgroup of eight waves (above) takes whole systems from a counter,
@@ -47,6 +49,8 @@ struct GemmP {  // element strides
 struct Tile {
     int p, m0, n0, pad;
 };
+
+typedef float mrm_f16 __attribute__((ext_vector_type(16)));
 
 __global__ void zero_counters_kernel(unsigned* c, int n) {
     if (int(threadIdx.x) < n) c[threadIdx.x] = 0;
@@ -152,6 +156,138 @@ __global__ void __launch_bounds__(BG_THREADS) bg_gemm_kernel(const GemmP* __rest
     }
 }
 
+// The same tile loop on the FP32 matrix cores with first-order error
+// compensation (the split of Ootomo and Yokota, "Recovering single precision
+// accuracy from Tensor Cores while surpassing the FP32 theoretical peak
+// performance", IJHPCA 2022, here with FP32 pieces of FP64 operands):
+// A = Ah + Al, B = Bh + Bl with Ah = fp32(A), Al = fp32(A - Ah), and
+// C = Ah Bh + (Ah Bl + Al Bh) with the neglected Al Bl below 2^-48 |A||B|.
+// Each K chunk of 16 accumulates the main and the correction products in
+// FP32 on v_mfma_f32_32x32x2_f32 (lane l supplies A[l%32][l/32] and
+// B[l/32][l%32]; result register v of lane l is row 8(v/4) + 4(l/32) + v%4,
+// column l%32), and the chunk sums are added into FP64 accumulators, so the
+// error is that of FP32 sums over 16 products, not over K.  T in {64, 128}:
+// four waves of (T/2) x (T/2) as (T/64)^2 blocks of 32 x 32.
+template <int T>
+__global__ void __launch_bounds__(BG_THREADS) bg_gemm_f32x3_kernel(const GemmP* __restrict__ probs,
+                                                                   const Tile* __restrict__ tiles, int ntiles,
+                                                                   unsigned* __restrict__ counter) {
+    static_assert(T == 64 || T == 128, "32 x 32 blocks per wave");
+    constexpr int BP = T / 64;
+    constexpr int EA = T * BG_BK / BG_THREADS;
+    constexpr int LDT = T + 4;
+    __shared__ float Ah[2][BG_BK][LDT], Al[2][BG_BK][LDT];
+    __shared__ float Bh[2][BG_BK][LDT], Bl[2][BG_BK][LDT];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    for (;;) {
+        if (tid == 0) s_item = int(atomicAdd(counter, 1u));
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= ntiles) return;
+        const Tile tl = tiles[item];
+        const GemmP pr = probs[tl.p];
+        const int64_t m0 = tl.m0, n0 = tl.n0;
+        double ra[EA], rb[EA];
+        const int ak = tid % BG_BK, ar = tid / BG_BK;
+        const int bc = tid % T, bk = tid / T;
+        const double* pa = pr.A + (m0 + ar) * pr.lda + ak;
+        const double* pb = pr.B + int64_t(bk) * pr.ldb + n0 + bc;
+        const bool bcol_ok = n0 + bc < pr.N;
+        auto fetch = [&](int64_t k0) {
+            const bool ak_ok = k0 + ak < pr.K;
+#pragma unroll
+            for (int i = 0; i < EA; ++i) {
+                const int rr = ar + (BG_THREADS / BG_BK) * i;
+                ra[i] = (ak_ok && m0 + rr < pr.M) ? pa[int64_t((BG_THREADS / BG_BK) * i) * pr.lda + k0] : 0.0;
+                const int kr = bk + (BG_THREADS / T) * i;
+                rb[i] = (bcol_ok && k0 + kr < pr.K) ? pb[(k0 + (BG_THREADS / T) * i) * pr.ldb] : 0.0;
+            }
+        };
+        auto stash = [&](int st) {
+#pragma unroll
+            for (int i = 0; i < EA; ++i) {
+                const int idx = tid + BG_THREADS * i;
+                const float ah = float(ra[i]), bh = float(rb[i]);
+                Ah[st][idx % BG_BK][idx / BG_BK] = ah;
+                Al[st][idx % BG_BK][idx / BG_BK] = float(ra[i] - double(ah));
+                Bh[st][idx / T][idx % T] = bh;
+                Bl[st][idx / T][idx % T] = float(rb[i] - double(bh));
+            }
+        };
+        double acc[BP][BP][16];
+#pragma unroll
+        for (int p = 0; p < BP; ++p)
+#pragma unroll
+            for (int c = 0; c < BP; ++c)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[p][c][v] = 0.0;
+        fetch(0);
+        stash(0);
+        __syncthreads();
+        int st = 0;
+        const int fr = lane & 31, fk = lane >> 5;
+        for (int64_t k0 = 0; k0 < pr.K; k0 += BG_BK) {
+            const bool more = k0 + BG_BK < pr.K;
+            if (more) fetch(k0 + BG_BK);
+            mrm_f16 hh[BP][BP], hc[BP][BP];
+#pragma unroll
+            for (int p = 0; p < BP; ++p)
+#pragma unroll
+                for (int c = 0; c < BP; ++c) {
+                    hh[p][c] = mrm_f16{};
+                    hc[p][c] = mrm_f16{};
+                }
+#pragma unroll
+            for (int kk = 0; kk < BG_BK / 2; ++kk) {
+                const int k = 2 * kk + fk;
+                float a_h[BP], a_l[BP], b_h[BP], b_l[BP];
+#pragma unroll
+                for (int p = 0; p < BP; ++p) {
+                    a_h[p] = Ah[st][k][wm * (T / 2) + 32 * p + fr];
+                    a_l[p] = Al[st][k][wm * (T / 2) + 32 * p + fr];
+                }
+#pragma unroll
+                for (int c = 0; c < BP; ++c) {
+                    b_h[c] = Bh[st][k][wn * (T / 2) + 32 * c + fr];
+                    b_l[c] = Bl[st][k][wn * (T / 2) + 32 * c + fr];
+                }
+#pragma unroll
+                for (int p = 0; p < BP; ++p)
+#pragma unroll
+                    for (int c = 0; c < BP; ++c) {
+                        hh[p][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_h[p], b_h[c], hh[p][c], 0, 0, 0);
+                        hc[p][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_h[p], b_l[c], hc[p][c], 0, 0, 0);
+                        hc[p][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_l[p], b_h[c], hc[p][c], 0, 0, 0);
+                    }
+            }
+#pragma unroll
+            for (int p = 0; p < BP; ++p)
+#pragma unroll
+                for (int c = 0; c < BP; ++c)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) acc[p][c][v] += double(hh[p][c][v]) + double(hc[p][c][v]);
+            if (more) stash(st ^ 1);
+            __syncthreads();
+            st ^= 1;
+        }
+#pragma unroll
+        for (int p = 0; p < BP; ++p)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int64_t row = m0 + wm * (T / 2) + 32 * p + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (row >= pr.M) continue;
+#pragma unroll
+                for (int c = 0; c < BP; ++c) {
+                    const int64_t col = n0 + wn * (T / 2) + 32 * c + (lane & 31);
+                    if (col < pr.N) pr.C[row * pr.ldc + col] = acc[p][c][v];
+                }
+            }
+    }
+}
+
 // Batched small LU solves: the kernels of mrm_batched_lu.h.
 using LuP = mrm::BatchLuP;
 
@@ -169,7 +305,7 @@ struct Launch {
 struct Plan {
     int kind = 0;          // 1 GEMM, 2 LU
     int tile = 0;          // GEMM tile chosen
-    int mode = 0;          // LU precision
+    int mode = 0;          // precision: MRM_SOLVE_FP64 or MRM_SOLVE_FP32_IR
     void* dev = nullptr;   // descriptors, tiles, counters (one allocation)
     unsigned* counters = nullptr;
     int ncounters = 0;
@@ -200,7 +336,10 @@ const void* gemm_fn() {
     return reinterpret_cast<const void*>(&bg_gemm_kernel<T>);
 }
 
-const void* gemm_fn_for(int t) {
+const void* gemm_fn_for(int t, int mode) {
+    if (mode == MRM_SOLVE_FP32_IR)
+        return t == 64 ? reinterpret_cast<const void*>(&bg_gemm_f32x3_kernel<64>)
+                       : reinterpret_cast<const void*>(&bg_gemm_f32x3_kernel<128>);
     return t == 32 ? gemm_fn<32>() : t == 64 ? gemm_fn<64>() : gemm_fn<128>();
 }
 
@@ -220,7 +359,7 @@ std::vector<Tile> make_tiles(const std::vector<GemmP>& probs, int t) {
 
 int launch_gemm(Plan& pl, int t, hipStream_t s) {
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, pl.counters, 1);
-    const void* fn = gemm_fn_for(t);
+    const void* fn = gemm_fn_for(t, pl.mode);
     const int grid = resident_grid(fn, BG_THREADS, 0);
     void* args[] = {&pl.probs, &pl.tiles, &pl.ntiles, &pl.counters};
     MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(BG_THREADS), args, 0, s));
@@ -281,11 +420,17 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 extern "C" {
 
 int32_t mrm_batched_dgemm_plan(const mrm_gemm_desc* problems, int64_t count, int32_t tile, void** plan) {
+    return mrm_batched_dgemm_plan_ex(problems, count, tile, MRM_SOLVE_FP64, plan);
+}
+
+int32_t mrm_batched_dgemm_plan_ex(const mrm_gemm_desc* problems, int64_t count, int32_t tile, int32_t mode,
+                                  void** plan) {
     int rc = mrm_ensure_init();
     if (rc) return rc;
+    const bool f32 = mode == MRM_SOLVE_FP32_IR;
     if (!plan || count < 0 || count > (1 << 24) || (count > 0 && !problems) ||
-        (tile != 0 && tile != 32 && tile != 64 && tile != 128)) {
-        mrm_set_error(MRM_BAD_ARG, "batched gemm: bad plan arguments");
+        (mode != MRM_SOLVE_FP64 && !f32) || (tile != 0 && tile != 64 && tile != 128 && (f32 || tile != 32))) {
+        mrm_set_error(MRM_BAD_ARG, "batched gemm: bad plan arguments (tiles 32/64/128 in FP64, 64/128 in FP32)");
         return MRM_BAD_ARG;
     }
     *plan = nullptr;
@@ -302,6 +447,7 @@ int32_t mrm_batched_dgemm_plan(const mrm_gemm_desc* problems, int64_t count, int
     }
     Plan* pl = new Plan;
     pl->kind = 1;
+    pl->mode = mode;
     std::lock_guard<std::mutex> lk(mrm_host_mutex());
     hipStream_t s = mrm_stream();
     // the largest tile list (T = 32) bounds the device storage
@@ -338,6 +484,7 @@ int32_t mrm_batched_dgemm_plan(const mrm_gemm_desc* problems, int64_t count, int
         (void)hipEventCreate(&e1);
         float best_ms = 1e30f;
         for (int t : {32, 64, 128}) {
+            if (f32 && t == 32) continue;
             if ((rc = use(t))) break;
             float tbest = 1e30f;
             for (int rep = 0; rep < 2 && !rc; ++rep) {

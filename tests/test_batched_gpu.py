@@ -82,6 +82,81 @@ def test_batched_gemm_full_size():
     plan.close()
 
 
+def _split_exact_batch(sizes, seed):
+    """Operands int + frac 2^-30 with 16 <= |int| <= 64 and |frac| <= 64: the
+    FP32 pieces are exactly the integer (frac 2^-30 is below half its FP32
+    ulp) and the fraction, every FP32 chunk sum is exact, so the compensated
+    product equals int*int + 2^-30 (int*frac + frac*int) exactly (only
+    frac*frac 2^-60 is dropped)."""
+    import torch
+    rng = np.random.default_rng(seed)
+
+    def ints(shape):
+        return rng.integers(16, 65, shape) * rng.choice([-1, 1], shape)
+
+    probs, want = [], []
+    for m, n, k in sizes:
+        ia, fa = ints((m, k)), rng.integers(-64, 65, (m, k))
+        ib, fb = ints((k, n)), rng.integers(-64, 65, (k, n))
+        A = ia + fa * 2.0 ** -30
+        B = ib + fb * 2.0 ** -30
+        X = (ia @ ib) * (1 << 30) + (ia @ fb + fa @ ib)   # exact in int64 (< 2^53)
+        want.append(X.astype(np.float64) * 2.0 ** -30)
+        probs.append((torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(),
+                      torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda")))
+    return probs, want
+
+
+@pytest.mark.parametrize("tile", [0, 64, 128])
+def test_batched_gemm_fp32_compensated_exact_pieces(tile):
+    """The FP32 mode forms Ah Bh + (Ah Bl + Al Bh): on operands whose pieces
+    and chunk sums are exact in FP32 it must return exactly that (a dropped
+    or mis-signed correction term differs by 2^-30 times an integer)."""
+    import torch
+    rng = np.random.default_rng(41)
+    sizes = [tuple(int(v) for v in rng.integers(1, 200, 3)) for _ in range(40)] + [(128, 128, 256), (1, 1, 1)]
+    probs, want = _split_exact_batch(sizes, 43)
+    plan = bt.dgemm_plan(probs, tile, _lib.SOLVE_FP32_IR)
+    assert plan.tile in (64, 128) and (tile == 0 or plan.tile == tile)
+    plan.run()
+    torch.cuda.synchronize()
+    for (A, B, Cm), w in zip(probs, want):
+        assert np.array_equal(Cm.cpu().numpy(), w)
+    plan.close()
+
+
+def test_batched_gemm_fp32_compensated_error():
+    """Random operands: every element within the bound of FP32 sums over K
+    chunks of 16, |C - A B| <= 17 2^-24 (|A||B|) (the dropped Al Bl and the
+    FP64 accumulation are far below it); the normwise error against the
+    oracle's FP64 GEMM is reported.  Graph replays are bit-identical."""
+    import torch
+    rng = np.random.default_rng(53)
+    sizes = [tuple(int(v) for v in rng.integers(16, 257, 3)) for _ in range(120)]
+    probs = _gemm_batch(sizes, 53)
+    plan = bt.dgemm_plan(probs, 0, _lib.SOLVE_FP32_IR)
+    plan.run()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for A, B, Cm in probs:
+        a, b, c = (t.cpu().numpy() for t in (A, B, Cm))
+        ref = orc.generic_mult(a, b)
+        bound = 17 * 2.0 ** -24 * (np.abs(a) @ np.abs(b))
+        assert np.all(np.abs(c - ref) <= bound)
+        worst = max(worst, orc.gemm_err(c, ref, a, b))
+    print(f"FP32 compensated batched GEMM: worst normwise error {worst:.3e} (FP64 bar {4 * 256 * U:.3e})")
+    assert worst > 4 * 256 * U * 1e-6  # a low-precision result, not an FP64 one
+    first = [p[2].clone() for p in probs]
+    for rep in range(2):
+        for p in probs:
+            p[2].fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.run()
+        torch.cuda.synchronize()
+        assert all(torch.equal(p[2], f) for p, f in zip(probs, first))
+    plan.close()
+
+
 def _lu_batch(ns, seed, singular=()):
     import torch
     g = torch.Generator(device="cuda").manual_seed(seed)
@@ -146,6 +221,8 @@ def test_batched_errors():
     h = C.c_void_p()
     assert lib.mrm_batched_dgemm_plan(None, 1, 0, C.byref(h)) == _lib.MRM_BAD_ARG
     assert lib.mrm_batched_dgemm_plan(None, 0, 48, C.byref(h)) == _lib.MRM_BAD_ARG
+    assert lib.mrm_batched_dgemm_plan_ex(None, 0, 32, _lib.SOLVE_FP32_IR, C.byref(h)) == _lib.MRM_BAD_ARG
+    assert lib.mrm_batched_dgemm_plan_ex(None, 0, 64, _lib.SOLVE_BF16_IR, C.byref(h)) == _lib.MRM_BAD_ARG
     d = (bt.LuDesc * 1)(bt.LuDesc(129, 1, 129 * 8, 1, 1))
     assert lib.mrm_batched_lu_plan(d, 1, 0, None, C.byref(h)) == _lib.MRM_BAD_DIMS
     assert lib.mrm_batched_lu_plan(d, 1, 2, None, C.byref(h)) == _lib.MRM_BAD_ARG

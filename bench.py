@@ -308,7 +308,7 @@ def ozaki_row(lib, reps: int = 3):
         "fastest_meeting_bar": {"slices": int(best), **per_s[best],
                                 "pct_of_fp64_peak": round(100 * per_s[best]["fp64_equiv_tflops"] /
                                                           FP64_PEAK_TFLOPS, 1)} if best else None,
-        "kernel": "oz_gemm_kernel (v_mfma_i32_32x32x32_i8), one launch per slice diagonal",
+        "kernel": "oz_fused_kernel (v_mfma_i32_32x32x32_i8), one launch per group of up to four slice diagonals",
     }
 
 

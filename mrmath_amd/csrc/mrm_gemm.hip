@@ -1,4 +1,4 @@
-// mrm_gemm.hip -- FP64 GEMM on the gfx950 matrix cores (v_mfma_f64_4x4x4_4b_f64).
+// mrm_gemm.hip -- FP64 GEMM on the gfx950 matrix cores (v_mfma_f64_16x16x4_f64).
 //
 // Replaces the reference's blocked FMA product (BlockMatrixMultiplication[T1/T2],
 // src/BlockedMult.pas:350-724, and its micro-kernel FMAMatrixMultTransposed,
@@ -10,47 +10,41 @@
 // is fused into the epilogue.
 //
 // Workgroup: 256 threads = 4 waves in a 2x2 arrangement; workgroup tile
-// BM x BN = 128 x 128, K step BK = 16; each wave owns a 64x64 block of C.  LDS
-// holds two stages of both panels (two workgroups per CU).  Interior K steps
-// move each panel straight from HBM into LDS with global_load_lds_dwordx4
-// (no register round trip, no LDS store instructions); the load of step t+1
-// is issued before step t is consumed and one barrier per step publishes it.
-// Edge tiles and partial K steps take a predicated register path, after the
-// step's compute, that writes the same LDS layout.
+// BM x BN = 128 x 128, K step BK = 16; each wave owns a 64x64 block of C as
+// 4x4 blocks of 16x16.  LDS holds two stages of both panels (two workgroups
+// per CU).  Interior K steps move each panel straight from HBM into LDS with
+// global_load_lds_dwordx4 (no register round trip, no LDS store
+// instructions); the load of step t+1 is issued before step t is consumed and
+// one barrier per step publishes it.  Edge tiles and partial K steps take a
+// predicated register path, after the step's compute, that writes the same
+// LDS layout.
 //
 // LDS panel layouts (r = the panel's M or N index):
 //   KC (operand K-contiguous in HBM): [r][k], 16 doubles per row, the eight
 //       16-byte chunks of row r stored at slot (chunk ^ (r % 8)), so that the
 //       16-byte fragment reads of eight consecutive rows hit distinct banks;
-//   RC (operand r-contiguous in HBM): [k][r], row stride SR = 132 doubles.
+//   RC (operand r-contiguous in HBM): [k][r], row stride SR = 136 doubles
+//       (k-rows 2 apart start 32 banks apart: the two lane groups of a
+//       64-lane 8-byte read touch disjoint banks).
 // One direct-load instruction of a wave writes 1 KiB of LDS (64 lanes x 16 B):
 // eight KC rows, or one RC k-row.
 //
-// Matrix instruction.  On gfx950 the four-block v_mfma_f64_4x4x4_4b_f64 runs
-// at the full FP64 rate (pure-MFMA loop: 77.6 of 78.6 TFLOP/s) while
-// v_mfma_f64_16x16x4_f64 tops out near 60-66.  Measured lane layout: lane
-// l = 16k + 4b + j supplies A_b[j][k] and B_b[k][j] of block b, and
-// D_b[i][n] lands in lane 16i + 4b + n.  The CBSZ/ABID/BLGP fields of FP64
-// MFMAs are negation modifiers on gfx950, so there is no operand broadcast:
-// every block carries its own B quad.  (Deriving the quads with DPP row
-// rotations instead of LDS reads was measured slower: FP64 VALU work competes
-// with FP64 MFMA issue.)
-// The wave tile is 4x4 sub-tiles (p, c) of 16x16.  The A fragment of sub-row
-// p puts row 16p + l%16 (= 16p + 4b + j) in lane l; the B fragment for column
-// quad q of sub-column c puts column 16c + 4q + l%4 in every block; block b
-// then yields C[16p + 4b + i][16c + 4q + n] in lane 16i + 4b + n, and four
-// quads cover the sub-tile.  The accumulator acc[p][c] element q therefore
-// holds row 16p + 4((l/4)%4) + l/16, column 16c + 4q + l%4.
+// Matrix instruction: v_mfma_f64_16x16x4_f64 (lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; result register v of lane l is row l/16 + 4v, column
+// l%16).  Each A and B value feeds 16 products per instruction, four times
+// the operand reuse of the four-block 4x4x4 form, so a K step reads 16 KiB of
+// fragments per wave from LDS instead of 40.  Pure-MFMA loops reach 76.6
+// TFLOP/s (97.5% of peak) with it, but only when the accumulators stay in
+// VGPRs: in the compiler's default AGPR form every accumulator is copied
+// between the register files around each instruction (36 TFLOP/s), so this
+// file is compiled with -mllvm -amdgpu-mfma-vgpr-form.
 //
 // K ordering inside a step: the MFMA's k index is the lane group g = l/16; a
 // step is two halves m, and in half m the first MFMA of a pair takes
 // k = 8m + 2g, the second k = 8m + 2g + 1, for A and B alike, so every
 // product A[m,k]*B[k,n] is formed exactly once and a lane's two k of a KC row
-// are one 16-byte chunk (see frag_pair).
-//
-// Pipelining: one half step m and one sub-column c form a group of 32 MFMAs;
-// the fragments of the next group are read from LDS while the current group
-// issues.
+// are one 16-byte chunk (see frag_pair).  Both halves' fragments are read at
+// the start of a step; the second half's land while the first half computes.
 
 #include "mrm_common.h"
 
@@ -60,7 +54,7 @@ constexpr int BM = 128;
 constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int NTHREADS = 256;
-constexpr int SR = BM + 4;                          // RC panels: [k][r] row stride in doubles
+constexpr int SR = BM + 8;                          // RC panels: [k][r] row stride in doubles
 constexpr int KC_PANEL = BM * BK;                   // KC panels: 128 rows x 16 doubles
 constexpr int RC_PANEL = BK * SR;
 constexpr int PANEL = KC_PANEL > RC_PANEL ? KC_PANEL : RC_PANEL;   // doubles per panel stage
@@ -278,9 +272,9 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     wait_vmem();
     __syncthreads();
 
-    // fragment rows of this lane: A row wm*64 + 16p + li, B column wn*64 + 16c + 4q + (lane & 3)
+    // fragment rows of this lane: A row wm*64 + 16p + li, B column wn*64 + 16c + li
     const int arow = wm * 64 + li;
-    const int bcol = wn * 64 + (lane & 3);
+    const int bcol = wn * 64 + li;
 
     for (int64_t t = 0; t < nk; ++t) {
         const int cur = int(t & 1);
@@ -299,39 +293,28 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         }
         const double* As = smem + cur * 2 * PANEL;
         const double* Bs = As + PANEL;
-        double2 a_cur[4], a_nxt[4], b_cur[4], b_nxt[4];
+        // both halves' fragments are read at once; the second half's land
+        // while the first half's 32 MFMAs run
+        double2 fa[2][4], fb[2][4];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) a_cur[p] = frag_pair<AKC>(As, arow + 16 * p, 0, lg);
+        for (int m = 0; m < 2; ++m) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) b_cur[q] = frag_pair<BKC>(Bs, bcol + 4 * q, 0, lg);
+            for (int p = 0; p < 4; ++p) fa[m][p] = frag_pair<AKC>(As, arow + 16 * p, m, lg);
 #pragma unroll
-        for (int gidx = 0; gidx < 8; ++gidx) {
-            const int c = gidx & 3;
-            if (gidx + 1 < 8) {
-                const int m2 = (gidx + 1) >> 2, c2 = (gidx + 1) & 3;
+            for (int c = 0; c < 4; ++c) fb[m][c] = frag_pair<BKC>(Bs, bcol + 16 * c, m, lg);
+        }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) b_nxt[q] = frag_pair<BKC>(Bs, bcol + 16 * c2 + 4 * q, m2, lg);
-                if (c2 == 0) {
-#pragma unroll
-                    for (int p = 0; p < 4; ++p) a_nxt[p] = frag_pair<AKC>(As, arow + 16 * p, m2, lg);
-                }
-            }
+        for (int m = 0; m < 2; ++m) {
 #pragma unroll
             for (int p = 0; p < 4; ++p)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    acc[p][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a_cur[p].x, b_cur[q].x, acc[p][c][q], 0, 0, 0);
+                for (int c = 0; c < 4; ++c)
+                    acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[m][p].x, fb[m][c].x, acc[p][c], 0, 0, 0);
 #pragma unroll
             for (int p = 0; p < 4; ++p)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    acc[p][c][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(a_cur[p].y, b_cur[q].y, acc[p][c][q], 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) b_cur[q] = b_nxt[q];
-            if (((gidx + 1) & 3) == 0) {
-#pragma unroll
-                for (int p = 0; p < 4; ++p) a_cur[p] = a_nxt[p];
-            }
+                for (int c = 0; c < 4; ++c)
+                    acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[m][p].y, fb[m][c].y, acc[p][c], 0, 0, 0);
         }
         if (a_regs) {
             la.load_regs();
@@ -345,41 +328,37 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         __syncthreads();
     }
 
-    // Epilogue: acc[p][c] element q of lane l is row 16p + 4((l/4)%4) + l/16,
-    // column 16c + 4q + l%4 of the wave tile.  For doAdd / doSub the 16 old
-    // values of a row group (and of their mirrors) are read together before
-    // any store: one memory round trip per row group instead of one per
-    // element (the compiler cannot reorder the reads past the stores).
-    const int erow = 4 * ((lane >> 2) & 3) + lg;
-    const int ecol = lane & 3;
+    // Epilogue: acc[p][c] element v of lane l is row 16p + l/16 + 4v, column
+    // 16c + l%16 of the wave tile (16 lanes write 128 contiguous bytes of a
+    // row).  For doAdd / doSub the old values of a row (and of their mirrors)
+    // are read together before any store: one memory round trip per row
+    // instead of one per element (the compiler cannot reorder the reads past
+    // the stores).
     const bool mirror = upper && tm != tn;  // the mirrored element of a symmetric product (M == N)
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-        const int64_t row = m0 + wm * 64 + 16 * p + erow;
-        if (row >= M) continue;
-        double* crow = C + row * ldc;
-        double old[4][4], mold[4][4];
-        if (op != MRM_DO_NONE) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = m0 + wm * 64 + 16 * p + lg + 4 * v;
+            if (row >= M) continue;
+            double* crow = C + row * ldc;
+            double old[4], mold[4];
+            if (op != MRM_DO_NONE) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int64_t col = n0 + wn * 64 + 16 * c + 4 * q + ecol;
-                    old[c][q] = col < N ? crow[col] : 0.0;
-                    mold[c][q] = (mirror && col < N) ? C[col * ldc + row] : 0.0;
+                for (int c = 0; c < 4; ++c) {
+                    const int64_t col = n0 + wn * 64 + 16 * c + li;
+                    old[c] = col < N ? crow[col] : 0.0;
+                    mold[c] = (mirror && col < N) ? C[col * ldc + row] : 0.0;
                 }
-        }
+            }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int64_t col = n0 + wn * 64 + 16 * c + 4 * q + ecol;
+            for (int c = 0; c < 4; ++c) {
+                const int64_t col = n0 + wn * 64 + 16 * c + li;
                 if (col < N) {
-                    const double x = acc[p][c][q];
-                    crow[col] = op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? old[c][q] + x : old[c][q] - x);
+                    const double x = acc[p][c][v];
+                    crow[col] = op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? old[c] + x : old[c] - x);
                     if (mirror)
-                        C[col * ldc + row] =
-                            op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? mold[c][q] + x : mold[c][q] - x);
+                        C[col * ldc + row] = op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? mold[c] + x : mold[c] - x);
                 }
             }
         }

@@ -330,35 +330,41 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
 
     // Epilogue: acc[p][c] element v of lane l is row 16p + l/16 + 4v, column
     // 16c + l%16 of the wave tile (16 lanes write 128 contiguous bytes of a
-    // row).  For doAdd / doSub the old values of a row (and of their mirrors)
-    // are read together before any store: one memory round trip per row
-    // instead of one per element (the compiler cannot reorder the reads past
-    // the stores).
+    // row).  For doAdd / doSub the 16 old values of a block row p (and of
+    // their mirrors) are read together before any of them is stored: one
+    // memory round trip per block row, four per tile (the compiler cannot
+    // move a read of C past a store to C; one round trip per row measured 35%
+    // of the LU's K = 256 trailing updates).
     const bool mirror = upper && tm != tn;  // the mirrored element of a symmetric product (M == N)
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
+        const int64_t r0 = m0 + wm * 64 + 16 * p + lg;
+        double old[4][4], mold[4][4];
+        if (op != MRM_DO_NONE) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t row = m0 + wm * 64 + 16 * p + lg + 4 * v;
-            if (row >= M) continue;
-            double* crow = C + row * ldc;
-            double old[4], mold[4];
-            if (op != MRM_DO_NONE) {
+            for (int v = 0; v < 4; ++v)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const int64_t col = n0 + wn * 64 + 16 * c + li;
-                    old[c] = col < N ? crow[col] : 0.0;
-                    mold[c] = (mirror && col < N) ? C[col * ldc + row] : 0.0;
+                    const int64_t row = r0 + 4 * v, col = n0 + wn * 64 + 16 * c + li;
+                    const bool in = row < M && col < N;
+                    old[v][c] = in ? C[row * ldc + col] : 0.0;
+                    mold[v][c] = (in && mirror) ? C[col * ldc + row] : 0.0;
                 }
-            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = r0 + 4 * v;
+            if (row >= M) continue;
+            double* crow = C + row * ldc;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int64_t col = n0 + wn * 64 + 16 * c + li;
                 if (col < N) {
                     const double x = acc[p][c][v];
-                    crow[col] = op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? old[c] + x : old[c] - x);
+                    crow[col] = op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? old[v][c] + x : old[v][c] - x);
                     if (mirror)
-                        C[col * ldc + row] = op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? mold[c] + x : mold[c] - x);
+                        C[col * ldc + row] =
+                            op == MRM_DO_NONE ? x : (op == MRM_DO_ADD ? mold[v][c] + x : mold[v][c] - x);
                 }
             }
         }

@@ -376,23 +376,55 @@ const void* lu_fn(int mode, int cls) {
     return cls == 2 ? mrm::batched_lu_block_kernel<1, 96>() : mrm::batched_lu_block_kernel<1, 128>();
 }
 
-int launch_lu(Plan& pl, hipStream_t s) {
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, pl.counters, Plan::NCLS);
+// The solve plan's graph is built node by node: the counter reset, then the
+// size classes' persistent kernels as independent branches, so the workgroups
+// of one class take over the CUs that another class's tail leaves idle.
+int build_lu_graph(Plan& pl) {
+    hipGraph_t g = nullptr;
+    MRM_HIP_TRY(hipGraphCreate(&g, 0));
+    auto fail = [&](hipError_t e) {
+        (void)hipGraphDestroy(g);
+        mrm_set_error(MRM_HIP_ERROR, "batched solve: graph construction failed: %s", hipGetErrorString(e));
+        return MRM_HIP_ERROR;
+    };
+    unsigned* ctrs = pl.counters;
+    int nc = Plan::NCLS;
+    void* zargs[] = {&ctrs, &nc};
+    hipKernelNodeParams zp{};
+    zp.func = reinterpret_cast<void*>(zero_counters_kernel);
+    zp.gridDim = dim3(1);
+    zp.blockDim = dim3(64);
+    zp.sharedMemBytes = 0;
+    zp.kernelParams = zargs;
+    zp.extra = nullptr;
+    hipGraphNode_t zero = nullptr;
+    hipError_t e = hipGraphAddKernelNode(&zero, g, nullptr, 0, &zp);
+    if (e != hipSuccess) return fail(e);
     for (int c = 0; c < Plan::NCLS; ++c) {
         if (pl.counts[c] == 0) continue;
         const void* fn = lu_fn(pl.mode, c);
         const int block = c < 2 ? mrm::BL_THREADS : mrm::BL_BLOCK_THREADS;
-        const int grid = resident_grid(fn, block, 0);
         unsigned* ctr = pl.counters + c;
         void* args[] = {&pl.sys[c], &pl.counts[c], &ctr, &pl.status, &pl.refine};
-        MRM_HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s));
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void*>(fn);
+        kp.gridDim = dim3(unsigned(resident_grid(fn, block, 0)));
+        kp.blockDim = dim3(unsigned(block));
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        if ((e = hipGraphAddKernelNode(&node, g, &zero, 1, &kp)) != hipSuccess) return fail(e);
     }
+    pl.graph = g;
+    MRM_HIP_TRY(hipGraphInstantiate(&pl.exec, g, nullptr, nullptr, 0));
     return MRM_OK;
 }
 
 int capture(Plan& pl, hipStream_t s) {
+    if (pl.kind == 2) return build_lu_graph(pl);
     MRM_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int rc = pl.kind == 1 ? launch_gemm(pl, pl.tile, s) : launch_lu(pl, s);
+    const int rc = launch_gemm(pl, pl.tile, s);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {

@@ -9,7 +9,7 @@
 
 namespace mrm {
 
-constexpr int LU_NB = 256;             // outer block: K of the trailing-update GEMM (FP64)
+constexpr int LU_NB = 512;             // outer block: K of the trailing-update GEMM (FP64)
 constexpr int LU_NB_LOWP = 512;        // outer block of the low-precision trailing update
 inline int lu_outer_block(int mode) { return mode == MRM_SOLVE_FP64 ? LU_NB : LU_NB_LOWP; }
 constexpr int LU_W = 32;               // panel width of the panel kernel

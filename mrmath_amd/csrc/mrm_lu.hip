@@ -6,9 +6,11 @@
 // unit-lower triangular solve LUBacksup (:138-169) and the Schur update through
 // the blocked multiply (:211-227).
 //
-// Factorisation: right-looking, blocked twice.  Outer blocks of LU_NB = 256
-// columns (512 for the low-precision trailing update) feed the GEMM with
-// K = 256 for the trailing update; each outer block is factored in panels of
+// Factorisation: right-looking, blocked twice.  Outer blocks of LU_NB = 512
+// columns (in every precision) feed the trailing-update GEMM with K = 512
+// (n = 16384: 172 ms against 175 ms with 256, whose 16-step K loops left the
+// tile prologue and the read-modify-write of C a larger share); each outer
+// block is factored in panels of
 // LU_W = 32 columns by the panel kernel (mrm_lu_panel.hip: its workgroups agree
 // on every pivot through flagged exchange words), with the reference's pivot
 // rule.  The pivot sequence is recorded both as indx
@@ -210,8 +212,6 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
     // no flag of an earlier factorisation may survive in the exchange area
     const int xw = int(lu_xch_bytes() / 4);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(grid1(xw, 256)), dim3(256), 0, s, static_cast<unsigned*>(ws.xch), xw);
-    // the low-precision path takes twice the outer block: half as many
-    // read-modify-write passes of the FP64 trailing matrix
     const int nb = lu_outer_block(mode);
     for (int J = 0; J < n; J += nb) {
         const int jb = n - J < nb ? n - J : nb;

@@ -24,6 +24,12 @@ int mrm_launch_dgemm_splitk(int trans, int64_t M, int64_t N, int64_t K, const do
 // The same with the shape's own K split and the partial products in the
 // caller's work: never allocates; a split that needs more than work_bytes
 // (or any split when work is NULL) is refused with MRM_BAD_DIMS.
+// `batch` products C_b (op)= op(A_b) op(B_b), X_b = X + b * sx (element strides
+// between the products; line widths in bytes), one launch, no K split.
+int mrm_launch_dgemm_batched(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, int64_t sa,
+                             const double* B, int64_t ldb, int64_t sb, double* C, int64_t ldc, int64_t sc, int batch,
+                             int op, hipStream_t stream);
+
 int mrm_launch_dgemm_ws(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                         int64_t ldb, double* C, int64_t ldc, int op, void* work, size_t work_bytes,
                         hipStream_t stream);

@@ -219,8 +219,15 @@ template <bool TA, bool TB, bool VEC>
 __global__ void __launch_bounds__(NTHREADS, 2)
 dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A, int64_t lda,
                   const double* __restrict__ B, int64_t ldb, double* __restrict__ C, int64_t ldc,
-                  int op, int tiles_m, int tiles_n, int64_t kchunk, int64_t cstride, int upper) {
+                  int op, int tiles_m, int tiles_n, int64_t kchunk, int64_t cstride, int upper, int64_t bsa,
+                  int64_t bsb, int64_t bsc) {
     __shared__ __attribute__((aligned(16))) double smem[2 * 2 * PANEL];
+    if (gridDim.z > 1) {
+        // strided batch: product blockIdx.z
+        A += int64_t(blockIdx.z) * bsa;
+        B += int64_t(blockIdx.z) * bsb;
+        C += int64_t(blockIdx.z) * bsc;
+    }
     if (gridDim.y > 1) {
         // split K: chunk blockIdx.y of kchunk (a multiple of BK) writes its
         // partial product to its own C slice
@@ -374,15 +381,16 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
 template <bool TA, bool TB>
 void launch_variant(bool vec, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                     int64_t ldb, double* C, int64_t ldc, int op, int tiles_m, int tiles_n, int upper, hipStream_t s,
-                    int splits = 1, int64_t kchunk = 0, int64_t cstride = 0) {
+                    int splits = 1, int64_t kchunk = 0, int64_t cstride = 0, int batch = 1, int64_t bsa = 0,
+                    int64_t bsb = 0, int64_t bsc = 0) {
     const unsigned ntiles = upper ? unsigned(tiles_m) * unsigned(tiles_m + 1) / 2 : unsigned(tiles_m) * unsigned(tiles_n);
-    dim3 grid(ntiles, unsigned(splits)), block(NTHREADS);
+    dim3 grid(ntiles, unsigned(splits), unsigned(batch)), block(NTHREADS);
     if (vec)
         hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n, kchunk, cstride, upper);
+                           op, tiles_m, tiles_n, kchunk, cstride, upper, bsa, bsb, bsc);
     else
         hipLaunchKernelGGL((dgemm_mfma_kernel<TA, TB, false>), grid, block, 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                           op, tiles_m, tiles_n, kchunk, cstride, upper);
+                           op, tiles_m, tiles_n, kchunk, cstride, upper, bsa, bsb, bsc);
 }
 
 // C (op)= sum_c W[c] over `splits` partial products, in chunk order.
@@ -522,6 +530,36 @@ int launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, in
 }
 
 }  // namespace
+
+int mrm_launch_dgemm_batched(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, int64_t sa,
+                             const double* B, int64_t ldb, int64_t sb, double* C, int64_t ldc, int64_t sc, int batch,
+                             int op, hipStream_t stream) {
+    if (M < 0 || N < 0 || K < 0 || op < 0 || op > 2 || trans < 0 || trans > 2 || batch < 0 || batch > 65535 ||
+        ((lda | ldb | ldc) & 7)) {
+        mrm_set_error(MRM_BAD_ARG, "dgemm batched: bad argument");
+        return MRM_BAD_ARG;
+    }
+    if (M == 0 || N == 0 || batch == 0) return MRM_OK;
+    if (K == 0) {
+        for (int b = 0; b < batch; ++b) {
+            const int rc = launch_dgemm(trans, M, N, 0, A, lda, B, ldb, C + int64_t(b) * sc, ldc, op, 1, nullptr, 0,
+                                        false, stream);
+            if (rc) return rc;
+        }
+        return MRM_OK;
+    }
+    const int64_t ea = lda / 8, eb = ldb / 8, ec = ldc / 8;
+    const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+    const bool vec = ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 &&
+                     ((lda | ldb) & 15) == 0 && ((sa | sb) & 1) == 0;
+    switch (trans) {
+        case MRM_NN: launch_variant<false, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), 0, stream, 1, 0, 0, batch, sa, sb, sc); break;
+        case MRM_TN: launch_variant<true, false>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), 0, stream, 1, 0, 0, batch, sa, sb, sc); break;
+        default:     launch_variant<false, true>(vec, M, N, K, A, ea, B, eb, C, ec, op, int(tm), int(tn), 0, stream, 1, 0, 0, batch, sa, sb, sc); break;
+    }
+    MRM_HIP_TRY(hipGetLastError());
+    return MRM_OK;
+}
 
 int mrm_launch_dgemm(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                      const double* B, int64_t ldb, double* C, int64_t ldc, int op, hipStream_t stream) {

@@ -339,7 +339,7 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
         A = (U * s) @ V.t()
         torch.cuda.synchronize()
         res = {}
-        for mode, name in ((0, "fp64"), (1, "fp32_ir"), (2, "bf16_ir")):
+        for mode, name in ((0, "fp64"), (1, "fp32_ir"), (2, "bf16_ir"), (3, "bf16x3_ir")):
             X = torch.zeros(n, nrhs, dtype=torch.float64, device=dev)
             st = _lib.SolveStats()
             ts = []
@@ -363,7 +363,7 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
                 res[name]["low_precision_attempt"] = {"steps": st.lowp_iterations,
                                                       "krylov_steps": st.lowp_krylov_steps,
                                                       "backward_error": st.lowp_backward_error}
-        for name in ("fp32_ir", "bf16_ir"):
+        for name in ("fp32_ir", "bf16_ir", "bf16x3_ir"):
             res[name]["speedup_vs_fp64"] = round(res["fp64"]["ms"] / res[name]["ms"], 3)
         out["kappa"][f"{kappa:g}"] = res
         del A
@@ -499,7 +499,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (uniform [-1,1) float64, device-resident)",
         "config": {"workload": workload, "M": M, "N": N, "K": K, "op": r["op"],
-                   "path": "mrm_dev_dgemm -> dgemm_mfma_kernel (v_mfma_f64_4x4x4_4b_f64)"},
+                   "path": "mrm_dev_dgemm -> dgemm_mfma_kernel (v_mfma_f64_16x16x4_f64)"},
         "pct_of_peak": round(100.0 * r["tflops"] / FP64_PEAK_TFLOPS, 2),
         "roofline": {
             "bound": "mfma",

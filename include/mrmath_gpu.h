@@ -62,10 +62,14 @@ extern "C" {
 #define MRM_NT 2 /* C = A * B^T (MultT2) */
 
 /* Factorisation precisions of the linear solvers: plain FP64, or the trailing
- * updates on the FP32 / BF16 matrix cores with FP64 iterative refinement. */
+ * updates on the FP32 / BF16 matrix cores with FP64 iterative refinement.
+ * MRM_SOLVE_BF16X3_IR splits each operand into two BF16 pieces, x = xh + xl,
+ * and forms Ah Bh + Ah Bl + Al Bh on the BF16 matrix cores (three products,
+ * error about 2^-16 instead of 2^-8 of one BF16 product). */
 #define MRM_SOLVE_FP64 0
 #define MRM_SOLVE_FP32_IR 1
 #define MRM_SOLVE_BF16_IR 2
+#define MRM_SOLVE_BF16X3_IR 3
 
 /* ---- library state ------------------------------------------------------ */
 
@@ -252,7 +256,8 @@ MRM_API int64_t mrm_lowp_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_
 
 /* C (M x N, FP64) -= A (M x K) * B (K x N), FP64 operands in device memory
  * (byte strides), the product rounded to FP32 (mode MRM_SOLVE_FP32_IR) or BF16
- * (MRM_SOLVE_BF16_IR) and accumulated in FP32 on the matrix cores.  The
+ * (MRM_SOLVE_BF16_IR; MRM_SOLVE_BF16X3_IR: two BF16 pieces per operand, three
+ * products) and accumulated in FP32 on the matrix cores.  The
  * Schur-complement update of the mixed-precision LU (the reference computes it
  * in FP64 through the blocked multiply, src/LinAlgLU.pas:211-227).
  * Asynchronous on `stream`; `work` must not be shared by calls in flight. */

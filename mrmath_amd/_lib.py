@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libmrmath_gpu.so"
  MRM_NOT_CONVERGED, MRM_TIMED_OUT) = range(9)
 DO_NONE, DO_ADD, DO_SUB = 0, 1, 2
 NN, TN, NT = 0, 1, 2
-SOLVE_FP64, SOLVE_FP32_IR, SOLVE_BF16_IR = 0, 1, 2
+SOLVE_FP64, SOLVE_FP32_IR, SOLVE_BF16_IR, SOLVE_BF16X3_IR = 0, 1, 2, 3
 
 _i64 = C.c_int64
 _i32 = C.c_int32

@@ -221,7 +221,8 @@ void ir_release_workspace() {
 int lineq_solve_ir(int mode, const double* A, int64_t lda, int n, const double* B, int64_t ldb, double* X,
                    int64_t ldx, int nrhs, int max_iter, double tol, IrStats* stats, hipStream_t s) {
     IrStats st;
-    if (mode != MRM_SOLVE_FP64 && mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR) {
+    if (mode != MRM_SOLVE_FP64 && mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR &&
+        mode != MRM_SOLVE_BF16X3_IR) {
         mrm_set_error(MRM_BAD_ARG, "solve: unknown mode %d", mode);
         return MRM_BAD_ARG;
     }

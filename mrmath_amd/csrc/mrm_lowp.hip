@@ -1,6 +1,11 @@
 // mrm_lowp.hip -- low-precision trailing update of the mixed-precision LU:
 // C (FP64) -= A * B with A (M x K) and B (K x N) rounded to FP32 or BF16 and
-// multiplied on the matrix cores with FP32 accumulation.
+// multiplied on the matrix cores with FP32 accumulation.  The split-BF16 mode
+// rounds each operand to a high BF16 piece and its remainder to a low one,
+// x = xh + xl + O(2^-16 |x|), and forms Ah Bh + Ah Bl + Al Bh (Al Bl, about
+// 2^-18 |A||B|, is dropped) as one BF16 product over three times K: the
+// pieces are packed one after the other along K, A as [Ah | Ah | Al] and B as
+// [Bh ; Bl ; Bh].
 //
 // The matrix being factored stays in FP64 (the panel, the pivot search and the
 // residuals of the refinement are FP64); only the Schur-complement product
@@ -45,7 +50,9 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 // Packs X (logical R x K, element (r, k) at X[r*rs + k*ks]) into 64-byte k
 // blocks.  64 rows x one k block per workgroup; reads coalesce along whichever
 // index is contiguous.
-template <int MODE>
+// PIECE (BF16): 0 the value rounded, 1 its low piece x - bf16(x) rounded (the
+// high piece is PIECE 0).
+template <int MODE, int PIECE = 0>
 __global__ void __launch_bounds__(256) lp_pack_kernel(const double* __restrict__ X, int64_t rs, int64_t ks, int64_t R,
                                                       int64_t K, uint8_t* __restrict__ P, int64_t Rp) {
     constexpr int EPB = MODE == MRM_LOWP_FP32 ? 16 : 32;  // elements per 64-byte block row
@@ -58,8 +65,13 @@ __global__ void __launch_bounds__(256) lp_pack_kernel(const double* __restrict__
         const int kk = kfast ? q % EPB : q / 64;
         const int64_t r = r0 + rr, k = k0 + kk;
         const double x = (r < R && k < K) ? X[r * rs + k * ks] : 0.0;
-        if (MODE == MRM_LOWP_FP32) *reinterpret_cast<float*>(&tile[rr][kk * ESZ]) = float(x);
-        else *reinterpret_cast<uint16_t*>(&tile[rr][kk * ESZ]) = to_bf16(float(x));
+        if (MODE == MRM_LOWP_FP32) {
+            *reinterpret_cast<float*>(&tile[rr][kk * ESZ]) = float(x);
+        } else {
+            const uint16_t h = to_bf16(float(x));
+            *reinterpret_cast<uint16_t*>(&tile[rr][kk * ESZ]) =
+                PIECE == 0 ? h : to_bf16(float(x - double(__uint_as_float(uint32_t(h) << 16))));
+        }
     }
     __syncthreads();
     const int rr = threadIdx.x / 4, c = threadIdx.x % 4;
@@ -224,7 +236,7 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 size_t mrm_lowp_workspace(int64_t M, int64_t N, int64_t K, int mode) {
     const int64_t epb = mode == MRM_LOWP_FP32 ? 16 : 32;
-    const int64_t kb = (K + epb - 1) / epb;
+    const int64_t kb = (K + epb - 1) / epb * (mode == MRM_LOWP_BF16X3 ? 3 : 1);
     return size_t(kb) * size_t(round_up(M, LP_TILE) + round_up(N, LP_TILE)) * 64;
 }
 
@@ -232,7 +244,7 @@ size_t mrm_lowp_workspace(int64_t M, int64_t N, int64_t K, int mode) {
 // (element strides), all FP64 in device memory; the product in `mode`.
 int mrm_launch_lowp_update(int mode, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
                            int64_t ldb, double* C, int64_t ldc, void* work, size_t work_bytes, hipStream_t stream) {
-    if (mode != MRM_LOWP_FP32 && mode != MRM_LOWP_BF16) {
+    if (mode != MRM_LOWP_FP32 && mode != MRM_LOWP_BF16 && mode != MRM_LOWP_BF16X3) {
         mrm_set_error(MRM_BAD_ARG, "lowp update: unknown mode %d", mode);
         return MRM_BAD_ARG;
     }
@@ -246,23 +258,41 @@ int mrm_launch_lowp_update(int mode, int64_t M, int64_t N, int64_t K, const doub
         return MRM_BAD_ARG;
     }
     const int64_t epb = mode == MRM_LOWP_FP32 ? 16 : 32;
-    const int64_t kb = (K + epb - 1) / epb;
+    const int64_t kb1 = (K + epb - 1) / epb;  // k blocks of one piece
+    const int64_t kb = kb1 * (mode == MRM_LOWP_BF16X3 ? 3 : 1);
     const int64_t Mp = round_up(M, LP_TILE), Np = round_up(N, LP_TILE);
     uint8_t* PA = static_cast<uint8_t*>(work);
     uint8_t* PB = PA + kb * Mp * 64;
-    const dim3 ga(unsigned(kb), unsigned(Mp / 64)), gb(unsigned(kb), unsigned(Np / 64));
+    const dim3 ga(unsigned(kb1), unsigned(Mp / 64)), gb(unsigned(kb1), unsigned(Np / 64));
     if (mode == MRM_LOWP_FP32) {
         hipLaunchKernelGGL(lp_pack_kernel<MRM_LOWP_FP32>, ga, dim3(256), 0, stream, A, lda, int64_t(1), M, K, PA, Mp);
         hipLaunchKernelGGL(lp_pack_kernel<MRM_LOWP_FP32>, gb, dim3(256), 0, stream, B, int64_t(1), ldb, N, K, PB, Np);
-    } else {
+    } else if (mode == MRM_LOWP_BF16) {
         hipLaunchKernelGGL(lp_pack_kernel<MRM_LOWP_BF16>, ga, dim3(256), 0, stream, A, lda, int64_t(1), M, K, PA, Mp);
         hipLaunchKernelGGL(lp_pack_kernel<MRM_LOWP_BF16>, gb, dim3(256), 0, stream, B, int64_t(1), ldb, N, K, PB, Np);
+    } else {
+        // three pieces along K, so that one BF16 product over 3K forms
+        // Ah Bh + Ah Bl + Al Bh: A as [Ah | Ah | Al], B as [Bh ; Bl ; Bh]
+        const int64_t sa = kb1 * Mp * 64, sb = kb1 * Np * 64;
+        hipLaunchKernelGGL((lp_pack_kernel<MRM_LOWP_BF16, 0>), ga, dim3(256), 0, stream, A, lda, int64_t(1), M, K, PA,
+                           Mp);
+        hipLaunchKernelGGL((lp_pack_kernel<MRM_LOWP_BF16, 0>), ga, dim3(256), 0, stream, A, lda, int64_t(1), M, K,
+                           PA + sa, Mp);
+        hipLaunchKernelGGL((lp_pack_kernel<MRM_LOWP_BF16, 1>), ga, dim3(256), 0, stream, A, lda, int64_t(1), M, K,
+                           PA + 2 * sa, Mp);
+        hipLaunchKernelGGL((lp_pack_kernel<MRM_LOWP_BF16, 0>), gb, dim3(256), 0, stream, B, int64_t(1), ldb, N, K, PB,
+                           Np);
+        hipLaunchKernelGGL((lp_pack_kernel<MRM_LOWP_BF16, 1>), gb, dim3(256), 0, stream, B, int64_t(1), ldb, N, K,
+                           PB + sb, Np);
+        hipLaunchKernelGGL((lp_pack_kernel<MRM_LOWP_BF16, 0>), gb, dim3(256), 0, stream, B, int64_t(1), ldb, N, K,
+                           PB + 2 * sb, Np);
     }
     const int tiles_m = int(Mp / (64 * LP_PM)), tiles_n = int(Np / (64 * LP_PN));
     const dim3 grid(unsigned(tiles_m) * unsigned(tiles_n));
     if (mode == MRM_LOWP_FP32)
         hipLaunchKernelGGL(lp_update_kernel<MRM_LOWP_FP32>, grid, dim3(LP_THREADS), 0, stream, PA, PB, Mp * 64,
                            Np * 64, kb, C, ldc, M, N, tiles_m, tiles_n);
+    // BF16 and split BF16: one BF16 product over kb k blocks
     else
         hipLaunchKernelGGL(lp_update_kernel<MRM_LOWP_BF16>, grid, dim3(LP_THREADS), 0, stream, PA, PB, Mp * 64,
                            Np * 64, kb, C, ldc, M, N, tiles_m, tiles_n);

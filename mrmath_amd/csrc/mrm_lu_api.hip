@@ -379,7 +379,9 @@ int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, cons
 }
 
 int64_t mrm_lowp_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_t mode) {
-    if (M < 0 || N < 0 || K < 0 || (mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR)) return 0;
+    if (M < 0 || N < 0 || K < 0 ||
+        (mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR && mode != MRM_SOLVE_BF16X3_IR))
+        return 0;
     return int64_t(mrm_lowp_workspace(M, N, K, mode));
 }
 
@@ -457,7 +459,7 @@ int32_t mrm_lineq_solve_ex(const double* A, int64_t LineWidthA, int64_t width, c
         mrm_set_error(MRM_BAD_DIMS, "Dimension error");
         return MRM_BAD_DIMS;
     }
-    if (mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR) {
+    if (mode != MRM_SOLVE_FP32_IR && mode != MRM_SOLVE_BF16_IR && mode != MRM_SOLVE_BF16X3_IR) {
         mrm_set_error(MRM_BAD_ARG, "solve: unknown mode %d", mode);
         return MRM_BAD_ARG;
     }

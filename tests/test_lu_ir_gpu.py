@@ -41,11 +41,12 @@ def _lowp_update(mode, A, B, C0):
     return dC.cpu().numpy()
 
 
-@pytest.mark.parametrize("mode", [_lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR])
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR, _lib.SOLVE_BF16X3_IR])
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 5), (300, 257, 256), (513, 260, 100)])
 def test_lowp_update_exact_on_small_integers(mode, M, N, K):
-    """Integers in [-8, 8] are exact in BF16 and FP32 and their sums of products
-    stay below 2^24, so C - A B must come out exactly."""
+    """Integers in [-8, 8] are exact in BF16 and FP32 (the low BF16 piece is 0)
+    and their sums of products stay below 2^24, so C - A B must come out
+    exactly."""
     rng = np.random.default_rng(M * N + K)
     A = rng.integers(-8, 9, (M, K)).astype(np.float64)
     B = rng.integers(-8, 9, (K, N)).astype(np.float64)
@@ -53,15 +54,21 @@ def test_lowp_update_exact_on_small_integers(mode, M, N, K):
     assert np.array_equal(_lowp_update(mode, A, B, C0), C0 - orc.generic_mult(A, B))
 
 
-@pytest.mark.parametrize("mode,u", [(_lib.SOLVE_FP32_IR, 2.0 ** -24), (_lib.SOLVE_BF16_IR, 2.0 ** -8)])
+@pytest.mark.parametrize("mode,u", [(_lib.SOLVE_FP32_IR, 2.0 ** -24), (_lib.SOLVE_BF16_IR, 2.0 ** -8),
+                                    (_lib.SOLVE_BF16X3_IR, 2.0 ** -16)])
 def test_lowp_update_rounding_bound(mode, u):
+    """|error| <= (2u + K' 2^-24) |A||B| elementwise: u the rounding of the
+    operands (for the split BF16 pieces: the dropped Al Bl and the rounding of
+    the low pieces, each about 2^-18), K' the FP32 sum's length (3K for the
+    split mode, whose three products run as one product over 3K)."""
     M, N, K = 700, 600, 256
     A = orc.pm1_matrix(K, M, 11)
     B = orc.pm1_matrix(N, K, 12)
     C0 = orc.pm1_matrix(N, M, 13)
     got = _lowp_update(mode, A, B, C0)
     ref = C0 - orc.generic_mult(A, B)
-    bound = (2 * u + K * 2.0 ** -24) * (np.abs(A) @ np.abs(B)) + 4 * 2.0 ** -53 * np.abs(C0)
+    kk = 3 * K if mode == _lib.SOLVE_BF16X3_IR else K
+    bound = (2 * u + kk * 2.0 ** -24) * (np.abs(A) @ np.abs(B)) + 4 * 2.0 ** -53 * np.abs(C0)
     err = np.abs(got - ref)
     assert np.all(err <= bound), float(np.max(err / bound))
     # and it is a low-precision product, not an FP64 one
@@ -92,7 +99,7 @@ def _solve_ir(mode, A, B, max_iter=50, tol=1e-14):
 
 
 @pytest.mark.parametrize("n,kappa,nrhs", [(257, 1e2, 3), (1000, 1e6, 8), (2500, 1e2, 64), (2500, 1e6, 64)])
-@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR])
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR, _lib.SOLVE_BF16X3_IR])
 def test_lu_ir_reaches_fp64_backward_error(n, kappa, nrhs, mode):
     A, B = _system(n, kappa, nrhs, int(n + np.log10(kappa)))
     rc, X, it, be, fb = _solve_ir(mode, A, B)
@@ -248,7 +255,7 @@ def system_16384():
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR])
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP64, _lib.SOLVE_FP32_IR, _lib.SOLVE_BF16_IR, _lib.SOLVE_BF16X3_IR])
 def test_n16384_64rhs_independent_residual(mode, system_16384):
     """The LU-IR workload itself (n = 16384, 64 right-hand sides, kappa = 1e2):
     the backward error is recomputed here from an independent FP64 residual

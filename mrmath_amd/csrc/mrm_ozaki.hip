@@ -40,7 +40,6 @@ typedef __attribute__((address_space(3))) void* lptr_t;
 
 constexpr int OZ_KB = 32;                 // bytes (= int8 k-values) per k block of the slice layout
 constexpr int OZ_KPAD = 64;               // K is padded to a multiple of this
-constexpr int OZ_THREADS = 256;
 
 // ---------------------------------------------------------------------------
 // Scaling exponents, stored biased: ev[r] = e + OZ_EBIAS with
@@ -149,8 +148,9 @@ __global__ void __launch_bounds__(256) oz_slice_kernel(const double* __restrict_
 // Every pair (t, u) with D0 <= t + u < D0 + ND uses slices t, u < T = D0 + ND
 // of either side, so one K step loads the T slice panels of A and of B once
 // and forms all the pairs from LDS, with one int32 accumulator set per
-// diagonal (exact: |sum| <= (d+1) KCHUNK 127^2 < 2^31).  Workgroup tile 128 x 128, four
-// waves in 2 x 2 of 64 x 64 (2 x 2 blocks of 32 x 32); K step 32 bytes (one
+// diagonal (exact: |sum| <= (d+1) KCHUNK 127^2 < 2^31).  Workgroup tile 128 x 128, eight
+// waves in 4 x 2 of 32 x 64 (1 x 2 blocks of 32 x 32) or four in 2 x 2 of
+// 64 x 64 (2 x 2 blocks); K step 32 bytes (one
 // v_mfma_i32_32x32x32_i8 k extent), NSTAGE-stage LDS ring
 // [stage][A slices, B slices][128 rows][32 B], the 16-byte chunk c of row r at
 // slot c ^ ((r/8) % 2) so a fragment read of 16 consecutive rows covers all
@@ -178,19 +178,22 @@ constexpr bool oz_a_used(int D0, int ND, int UH, int h, int t) {
     return false;
 }
 
-template <int D0, int ND, int NSTAGE>
-__global__ void __launch_bounds__(OZ_THREADS, 1)
+template <int D0, int ND, int NSTAGE, int NWAVE>
+__global__ void __launch_bounds__(NWAVE * 64, 1)
 oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, int64_t kbpa, int64_t kbpb, int64_t spa,
                 int64_t spb, int64_t kstep0, int64_t nk, const int* __restrict__ ea, const int* __restrict__ eb,
                 double* __restrict__ C, int64_t ldc, int64_t M, int64_t N, int op, int tiles_m, int tiles_n) {
+    static_assert(NWAVE == 4 || NWAVE == 8, "2 x 2 waves of 64 x 64, or 4 x 2 of 32 x 64");
     constexpr int T = D0 + ND;              // slices 0..T-1 of each operand take part
     constexpr int RB = OZ_KB;               // bytes of k per row and step
     constexpr int PANEL = 128 * RB;         // one slice of one operand, one step
     constexpr int STAGE = 2 * T * PANEL;
+    constexpr int PBA = NWAVE == 4 ? 2 : 1; // 32-row A blocks per wave (B: two 32-column blocks)
+    constexpr int LPS = NWAVE == 4 ? 2 * T : T;  // direct loads per thread and step
     __shared__ __attribute__((aligned(16))) int8_t smem[NSTAGE * STAGE];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = NWAVE == 4 ? (wave & 1) : (wave & 3), wn = NWAVE == 4 ? (wave >> 1) : (wave >> 2);
     int tm, tn;
     {
         // workgroups are dealt round-robin to the 8 XCDs: XCD x gets a
@@ -211,10 +214,13 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
     }
     const int64_t m0 = int64_t(tm) * 128, n0 = int64_t(tn) * 128;
 
-    // direct loads: one instruction of wave w writes rows 32w .. 32w+31 of a
-    // panel (1 KiB); lane L covers row 32w + L/2, LDS slot L%2 <- HBM chunk
-    // (L%2) ^ ((L/16)%2) (= the slot rule for that row)
-    const int lrow = 32 * wave + (lane >> 1);
+    // direct loads: one instruction of a wave writes rows 32q .. 32q+31 of a
+    // panel (1 KiB), q = wave % 4; lane L covers row 32q + L/2, LDS slot L%2 <-
+    // HBM chunk (L%2) ^ ((L/16)%2) (= the slot rule for that row).  Four waves:
+    // each loads its rows of every panel; eight: waves 0-3 the A panels, 4-7
+    // the B panels.
+    const int lq = wave & 3;
+    const int lrow = 32 * lq + (lane >> 1);
     const int gch = (lane & 1) ^ ((lane >> 4) & 1);
     const int8_t* pa = SA + kstep0 * kbpa + (m0 + lrow) * OZ_KB + 16 * gch;
     const int8_t* pb = SB + kstep0 * kbpb + (n0 + lrow) * OZ_KB + 16 * gch;
@@ -223,20 +229,25 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
     auto issue = [&](int stage, int64_t st) {
         const int64_t oa = st * kbpa;
         const int64_t ob = st * kbpb;
-        int8_t* l = smem + stage * STAGE + wave * (32 * RB);
+        int8_t* l = smem + stage * STAGE + lq * (32 * RB);
+        if (NWAVE == 4 || wave < 4) {
 #pragma unroll
-        for (int t = 0; t < T; ++t)
-            __builtin_amdgcn_global_load_lds((gptr_t)(pa + t * spa + oa), (lptr_t)(l + t * PANEL), 16, 0, 0);
+            for (int t = 0; t < T; ++t)
+                __builtin_amdgcn_global_load_lds((gptr_t)(pa + t * spa + oa), (lptr_t)(l + t * PANEL), 16, 0, 0);
+        }
+        if (NWAVE == 4 || wave >= 4) {
 #pragma unroll
-        for (int u = 0; u < T; ++u)
-            __builtin_amdgcn_global_load_lds((gptr_t)(pb + u * spb + ob), (lptr_t)(l + (T + u) * PANEL), 16, 0, 0);
+            for (int u = 0; u < T; ++u)
+                __builtin_amdgcn_global_load_lds((gptr_t)(pb + u * spb + ob), (lptr_t)(l + (T + u) * PANEL), 16, 0,
+                                                 0);
+        }
     };
 
-    mrm_i16 acc[ND][2][2];
+    mrm_i16 acc[ND][PBA][2];
 #pragma unroll
     for (int e = 0; e < ND; ++e)
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < PBA; ++p)
 #pragma unroll
             for (int c = 0; c < 2; ++c) acc[e][p][c] = mrm_i16{};
 
@@ -246,14 +257,14 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
     auto frag = [&](const int8_t* P, int r) {
         return *reinterpret_cast<const mrm_i4*>(P + r * RB + 16 * (fch ^ ((r >> 3) & 1)));
     };
-    const int ra0 = wm * 64 + frow, rb0 = wn * 64 + frow;
+    const int ra0 = wm * (32 * PBA) + frow, rb0 = wn * 64 + frow;
 
     // Wait until this thread's loads of the current step have landed while
-    // those of `later` further steps stay in flight (2T instructions a step).
-    static_assert(NSTAGE >= 2 && NSTAGE <= 3 && 2 * 2 * T < 64, "vmcnt range");
+    // those of `later` further steps stay in flight (LPS instructions a step).
+    static_assert(NSTAGE >= 2 && NSTAGE <= 3 && 2 * LPS < 64, "vmcnt range");
     auto wait_loads = [&](int64_t later) {
-        if (NSTAGE >= 3 && later >= 2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * 2 * T) : "memory");
-        else if (later >= 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * T) : "memory");
+        if (NSTAGE >= 3 && later >= 2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(2 * LPS) : "memory");
+        else if (later >= 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(LPS) : "memory");
         else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     };
     auto barrier = [&]() {
@@ -268,15 +279,15 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
     // next step's half-0 fragments while half 1 computes (the MFMAs keep
     // running across the barrier).
     constexpr int UH = oz_split_u(D0, ND);
-    mrm_i4 fa[2][T][2], fb[2][T][2];
+    mrm_i4 fa[2][T][PBA], fb[2][T][2];
     auto load_half = [&](int stage, int h) {
         const int8_t* As = smem + stage * STAGE;
         const int8_t* Bs = As + T * PANEL;
 #pragma unroll
         for (int t = 0; t < T; ++t)
             if (oz_a_used(D0, ND, UH, h, t)) {
-                fa[h][t][0] = frag(As + t * PANEL, ra0);
-                fa[h][t][1] = frag(As + t * PANEL, ra0 + 32);
+#pragma unroll
+                for (int p = 0; p < PBA; ++p) fa[h][t][p] = frag(As + t * PANEL, ra0 + 32 * p);
             }
 #pragma unroll
         for (int u = 0; u < T; ++u)
@@ -294,7 +305,7 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
                 if (!oz_pair(D0, ND, t, u)) continue;
                 const int e = t + u - D0;
 #pragma unroll
-                for (int p = 0; p < 2; ++p)
+                for (int p = 0; p < PBA; ++p)
 #pragma unroll
                     for (int c = 0; c < 2; ++c)
                         acc[e][p][c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[h][t][p], fb[h][u][c], acc[e][p][c],
@@ -334,10 +345,10 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
     // column l%32 of its block.
     const int later = op == MRM_DO_SUB ? MRM_DO_SUB : MRM_DO_ADD;
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < PBA; ++p) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const int64_t row = m0 + wm * 64 + 32 * p + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            const int64_t row = m0 + wm * (32 * PBA) + 32 * p + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
             if (row >= M) continue;
             const int er = oz_exponent(ea[row]);
             double* crow = C + row * ldc;
@@ -362,9 +373,15 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
 // Three LDS stages while they fit in the 160 KiB of a CU, else two.
 constexpr int oz_fstages(int T) { return 3 * 2 * T * 128 * 32 <= 160 * 1024 ? 3 : 2; }
 
+// Waves per workgroup: eight (32 x 64 each, two per SIMD; measured 2-19%
+// faster per group at 8192^3) except for (3, 4), whose eight-wave form spills
+// registers (10.57 against 10.24 ms with four waves of 64 x 64).
+template <int D0, int ND>
+constexpr int oz_waves() { return (D0 == 3 && ND == 4) ? 4 : 8; }
+
 template <int D0, int ND>
 const void* oz_fused_fn() {
-    return reinterpret_cast<const void*>(&oz_fused_kernel<D0, ND, oz_fstages(D0 + ND)>);
+    return reinterpret_cast<const void*>(&oz_fused_kernel<D0, ND, oz_fstages(D0 + ND), oz_waves<D0, ND>()>);
 }
 
 // Diagonal groups per slice count, one launch each.  A group of ND diagonals
@@ -391,16 +408,22 @@ constexpr OzGroup OZ_GROUPS[MRM_OZAKI_MAX_SLICES + 1][OZ_MAX_GROUPS] = {
 };
 static_assert(MRM_OZAKI_MAX_SLICES == 8, "a group table row per slice count");
 
-const void* oz_fused_for(int d0, int nd) {
+// The kernel of group (d0, nd) and its workgroup size.
+const void* oz_fused_for(int d0, int nd, int& threads) {
     switch (d0 * 8 + nd) {
-        case 0 * 8 + 1: return oz_fused_fn<0, 1>();
-        case 0 * 8 + 2: return oz_fused_fn<0, 2>();
-        case 0 * 8 + 3: return oz_fused_fn<0, 3>();
-        case 0 * 8 + 4: return oz_fused_fn<0, 4>();
-        case 2 * 8 + 3: return oz_fused_fn<2, 3>();
-        case 2 * 8 + 4: return oz_fused_fn<2, 4>();
-        case 3 * 8 + 4: return oz_fused_fn<3, 4>();
-        case 7 * 8 + 1: return oz_fused_fn<7, 1>();
+#define OZ_CASE(a, b)                      \
+    case a * 8 + b:                        \
+        threads = 64 * oz_waves<a, b>();   \
+        return oz_fused_fn<a, b>();
+        OZ_CASE(0, 1)
+        OZ_CASE(0, 2)
+        OZ_CASE(0, 3)
+        OZ_CASE(0, 4)
+        OZ_CASE(2, 3)
+        OZ_CASE(2, 4)
+        OZ_CASE(3, 4)
+        OZ_CASE(7, 1)
+#undef OZ_CASE
         default: return nullptr;
     }
 }
@@ -476,7 +499,8 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
             const int d0 = OZ_GROUPS[slices][g].d0, nd = OZ_GROUPS[slices][g].nd;
             int o = first ? op : (op == MRM_DO_SUB ? MRM_DO_SUB : MRM_DO_ADD);
             first = false;
-            const void* fn = oz_fused_for(d0, nd);
+            int threads = 0;
+            const void* fn = oz_fused_for(d0, nd, threads);
             if (!fn) {
                 mrm_set_error(MRM_BAD_ARG, "ozaki: no kernel for diagonals %d..%d", d0, d0 + nd - 1);
                 return MRM_BAD_ARG;
@@ -490,8 +514,8 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
             int64_t spav = spa, spbv = spb;
             double* cv = C;
             void* args[] = {&sav, &sbv, &kbpa, &kbpb, &spav, &spbv, &kk, &nk, &eav, &ebv, &cv, &ecv, &Mv, &Nv, &o, &tmv, &tnv};
-            MRM_HIP_TRY(hipLaunchKernel(fn, dim3(unsigned(tiles_m) * unsigned(tiles_n)), dim3(OZ_THREADS), args, 0,
-                                        stream));
+            MRM_HIP_TRY(hipLaunchKernel(fn, dim3(unsigned(tiles_m) * unsigned(tiles_n)), dim3(unsigned(threads)),
+                                        args, 0, stream));
         }
     }
     MRM_HIP_TRY(hipGetLastError());

@@ -26,6 +26,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "mrm_common.h"
 #include "mrm_host.h"
@@ -158,24 +159,57 @@ __global__ void __launch_bounds__(256) oz_slice_kernel(const double* __restrict_
 // FP64 roundings as one launch per diagonal.
 // ---------------------------------------------------------------------------
 constexpr bool oz_pair(int D0, int ND, int t, int u) { return t + u >= D0 && t + u < D0 + ND; }
-// smallest UH with at least half of the group's pairs at u < UH
+// largest UH with at least half of the group's pairs at u >= UH (half 1, the
+// half computed after the barrier, while the global loads and the next
+// step's half-0 reads issue)
 constexpr int oz_split_u(int D0, int ND) {
     const int T = D0 + ND;
     int total = 0;
     for (int u = 0; u < T; ++u)
         for (int t = 0; t < T; ++t) total += oz_pair(D0, ND, t, u) ? 1 : 0;
     int acc = 0;
-    for (int u = 0; u < T; ++u) {
+    for (int u = T - 1; u > 0; --u) {
         for (int t = 0; t < T; ++t) acc += oz_pair(D0, ND, t, u) ? 1 : 0;
-        if (2 * acc >= total) return u + 1;
+        if (2 * acc >= total) return u;
     }
-    return T;
+    return 1;
 }
 // slice t of A takes part in a pair of half h (0: u < UH, 1: u >= UH)
 constexpr bool oz_a_used(int D0, int ND, int UH, int h, int t) {
     for (int u = h ? UH : 0; u < (h ? D0 + ND : UH); ++u)
         if (oz_pair(D0, ND, t, u)) return true;
     return false;
+}
+// fragment reads (ds_read_b128) and MFMAs of half h for PBA 32-row A blocks
+constexpr int oz_half_reads(int D0, int ND, int PBA, int h) {
+    const int UH = oz_split_u(D0, ND), T = D0 + ND;
+    int r = 0;
+    for (int t = 0; t < T; ++t) r += oz_a_used(D0, ND, UH, h, t) ? PBA : 0;
+    for (int u = 0; u < T; ++u) r += ((u < UH) == (h == 0)) ? 2 : 0;
+    return r;
+}
+constexpr int oz_half_mfmas(int D0, int ND, int PBA, int h) {
+    const int UH = oz_split_u(D0, ND), T = D0 + ND;
+    int m = 0;
+    for (int u = 0; u < T; ++u)
+        if ((u < UH) == (h == 0))
+            for (int t = 0; t < T; ++t) m += oz_pair(D0, ND, t, u) ? 2 * PBA : 0;
+    return m;
+}
+// Instruction order of one half: MFMA, then up to one global load and up to
+// two fragment reads, repeated; the last MFMA takes what is left.  Issuing
+// the reads and loads as one burst ahead of the MFMAs stalled the waves on a
+// full LDS queue with the matrix pipe idle (8-12% slower per group at 8192^3).
+template <int NV, int NR, int NM>
+__device__ __forceinline__ void oz_interleave() {
+    if constexpr (NM > 0) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        constexpr int V = NM == 1 ? NV : (NV > 0 ? 1 : 0);
+        constexpr int R = NM == 1 ? NR : (NR < 2 ? NR : 2);
+        if constexpr (V > 0) __builtin_amdgcn_sched_group_barrier(0x010, V, 0);  // VMEM
+        if constexpr (R > 0) __builtin_amdgcn_sched_group_barrier(0x100, R, 0);  // DS read
+        oz_interleave<NV - V, NR - R, NM - 1>();
+    }
 }
 
 template <int D0, int ND, int NSTAGE, int NWAVE>
@@ -320,21 +354,30 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
     wait_loads(steps - 1 < NSTAGE - 1 ? steps - 1 : NSTAGE - 1);
     barrier();
     load_half(0, 0);
-    for (int64_t st = 0; st + 1 < steps; ++st) {
+    constexpr int R0 = oz_half_reads(D0, ND, PBA, 0), R1 = oz_half_reads(D0, ND, PBA, 1);
+    constexpr int M0 = oz_half_mfmas(D0, ND, PBA, 0), M1 = oz_half_mfmas(D0, ND, PBA, 1);
+    // one K step; `refill` (compile time) reloads the stage just consumed
+    // with step st + NSTAGE, so the steps that refill and the last NSTAGE - 1
+    // that do not run as two loops without a branch inside
+    auto step = [&](int64_t st, auto refill) {
+        constexpr bool kRefill = decltype(refill)::value;
         const int cur = int(st % NSTAGE);
         __builtin_amdgcn_s_waitcnt(0xC07F);
         load_half(cur, 1);
-        __builtin_amdgcn_sched_barrier(0);
         mma_half(0);
+        oz_interleave<0, R1, M0>();
         // steps issued so far: up to min(steps-1, st+NSTAGE-1)
         const int64_t last = steps - 1 < st + NSTAGE - 1 ? steps - 1 : st + NSTAGE - 1;
         wait_loads(last - (st + 1));
         barrier();
-        if (st + NSTAGE < steps) issue(cur, st + NSTAGE);
+        if constexpr (kRefill) issue(cur, st + NSTAGE);
         load_half(int((st + 1) % NSTAGE), 0);
-        __builtin_amdgcn_sched_barrier(0);
         mma_half(1);
-    }
+        oz_interleave<kRefill ? LPS : 0, R0, M1>();
+    };
+    int64_t st = 0;
+    for (; st + NSTAGE < steps; ++st) step(st, std::true_type());
+    for (; st + 1 < steps; ++st) step(st, std::false_type());
     __builtin_amdgcn_s_waitcnt(0xC07F);
     load_half(int((steps - 1) % NSTAGE), 1);
     mma_half(0);
@@ -373,9 +416,9 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
 // Three LDS stages while they fit in the 160 KiB of a CU, else two.
 constexpr int oz_fstages(int T) { return 3 * 2 * T * 128 * 32 <= 160 * 1024 ? 3 : 2; }
 
-// Waves per workgroup: eight (32 x 64 each, two per SIMD; measured 2-19%
-// faster per group at 8192^3) except for (3, 4), whose eight-wave form spills
-// registers (10.57 against 10.24 ms with four waves of 64 x 64).
+// Waves per workgroup: eight (32 x 64 each, two per SIMD; 1-6% faster per
+// group at 8192^3) except for (3, 4), whose eight-wave form spills registers
+// in its K loop (four waves of 64 x 64 time the same, 9.7 ms).
 template <int D0, int ND>
 constexpr int oz_waves() { return (D0 == 3 && ND == 4) ? 4 : 8; }
 
@@ -385,12 +428,13 @@ const void* oz_fused_fn() {
 }
 
 // Diagonal groups per slice count, one launch each.  A group of ND diagonals
-// holds ND int32 accumulator sets of 64 registers per lane, so ND <= 4 (five
-// and six spill); within that, larger groups load fewer slice panels per pair.
-// Measured at 8192^3 per group (ms): (0,4) 5.26 against (0,3)+(3,1) 6.65 and
-// (0,2)+(2,2) 6.37; s = 5: (0,2)+(2,3) 8.38 against (0,3)+(3,2) 8.78; s = 6:
-// (0,2)+(2,4) 11.11 against (0,3)+(3,3) 11.57 and (0,4)+(4,2) 11.94; s = 7:
-// (0,3)+(3,4) 14.18 against (0,4)+(4,3) 14.52.
+// holds ND int32 accumulator sets of 64 registers per lane (four waves) or 32
+// (eight), so ND <= 4 (five and six spill); within that, larger groups load
+// fewer slice panels per pair.  Chosen from per-group times at 8192^3 (ms,
+// four-wave kernels without the interleaved schedule): (0,4) 5.26 against
+// (0,3)+(3,1) 6.65 and (0,2)+(2,2) 6.37; s = 5: (0,2)+(2,3) 8.38 against
+// (0,3)+(3,2) 8.78; s = 6: (0,2)+(2,4) 11.11 against (0,3)+(3,3) 11.57 and
+// (0,4)+(4,2) 11.94; s = 7: (0,3)+(3,4) 14.18 against (0,4)+(4,3) 14.52.
 struct OzGroup {
     int d0, nd;
 };

@@ -256,9 +256,9 @@ def covariance_row(lib, h: int = 200000, w: int = 512, reps: int = 10):
 
 
 def ozaki_row(lib, reps: int = 3):
-    """FP64 GEMM emulated with s = 3..7 INT8 slices at 8192^3 on wide-range
-    inputs, against the native FP64 MFMA GEMM of the same operands (itself
-    checked against the oracle in the tests)."""
+    """FP64 GEMM emulated with s = 3..7 INT8 slices (and s = 6, 7 BF16
+    slices) at 8192^3 on wide-range inputs, against the native FP64 MFMA GEMM
+    of the same operands (itself checked against the oracle in the tests)."""
     import torch
 
     n = 8192
@@ -301,6 +301,36 @@ def ozaki_row(lib, reps: int = 3):
                          "normwise_err_vs_fp64_gemm": err, "meets_fp64_bar": err <= bar}
     ok = [s for s in per_s if per_s[s]["meets_fp64_bar"]]
     best = max(ok, key=lambda s: per_s[s]["fp64_equiv_tflops"]) if ok else None
+    # the same GEMMs from BF16 digits (v_mfma_f32_32x32x16_bf16, exact FP32
+    # sums over 128 k): bit-equal to the INT8 result, at half the INT8 rate
+    wbb = lib.mrm_ozaki_workspace_bytes_ex(n, n, n, 7, 1)
+    del W
+    W = torch.empty(wbb, dtype=torch.uint8, device=dev)
+    C8 = torch.empty_like(Cd)
+    per_bf = {}
+    for s in (6, 7):
+        assert lib.mrm_dev_dgemm_ozaki(0, n, n, n, A.data_ptr(), n * 8, B.data_ptr(), n * 8, C8.data_ptr(), n * 8,
+                                       0, s, W.data_ptr(), wbb, sp) == 0
+
+        def call_bf():
+            rc = lib.mrm_dev_dgemm_ozaki_ex(0, n, n, n, A.data_ptr(), n * 8, B.data_ptr(), n * 8, C.data_ptr(),
+                                            n * 8, 0, s, 1, W.data_ptr(), wbb, sp)
+            assert rc == 0, rc
+        call_bf()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call_bf()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = statistics.median(ts)
+        per_bf[str(s)] = {"ms": round(ms, 3), "fp64_equiv_tflops": round(2.0 * n ** 3 / ms / 1e9, 2),
+                          "bf16_tflops": round(s * (s + 1) // 2 * 2.0 * n ** 3 / ms / 1e9, 1),
+                          "normwise_err_vs_fp64_gemm": float(torch.linalg.norm(C - Cd)) / na,
+                          "bit_equal_to_int8_slices": bool(torch.equal(C, C8))}
     return {
         "config": "8192^3, entries (2 res53 - 1) 2^e, e uniform in [-20, 20]",
         "bar": f"||C - C_fp64||_F / (||A||_F ||B||_F) <= 4 k 2^-53 = {bar:.3e}",
@@ -309,6 +339,7 @@ def ozaki_row(lib, reps: int = 3):
                                 "pct_of_fp64_peak": round(100 * per_s[best]["fp64_equiv_tflops"] /
                                                           FP64_PEAK_TFLOPS, 1)} if best else None,
         "kernel": "oz_fused_kernel (v_mfma_i32_32x32x32_i8), one launch per group of up to four slice diagonals",
+        "bf16_slices": per_bf,
     }
 
 

@@ -202,6 +202,23 @@ MRM_API int32_t mrm_dev_dgemm_ozaki(int32_t trans, int64_t M, int64_t N, int64_t
                                     const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, int32_t slices,
                                     void* work, int64_t work_bytes, void* stream);
 
+/* Slice kinds of mrm_dev_dgemm_ozaki_ex: signed 7-bit digits held as INT8
+ * (v_mfma_i32_32x32x32_i8, int32 sums) or as BF16 (v_mfma_f32_32x32x16_bf16,
+ * FP32 sums over 128 k, exact, added into int32).  Both give the same digits
+ * and the same exact diagonal sums, so their results are equal bit for bit. */
+#define MRM_OZAKI_INT8 0
+#define MRM_OZAKI_BF16 1
+
+/* Device bytes of workspace mrm_dev_dgemm_ozaki_ex needs (BF16 slices take
+ * two bytes a digit); 0 for a bad size, slice count or kind. */
+MRM_API int64_t mrm_ozaki_workspace_bytes_ex(int64_t M, int64_t N, int64_t K, int32_t slices, int32_t kind);
+
+/* mrm_dev_dgemm_ozaki with the slice kind chosen (MRM_OZAKI_INT8 or
+ * MRM_OZAKI_BF16); the same GEMM, arguments and error bound. */
+MRM_API int32_t mrm_dev_dgemm_ozaki_ex(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                                       const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op,
+                                       int32_t slices, int32_t kind, void* work, int64_t work_bytes, void* stream);
+
 /* Host-pointer form of mrm_dev_dgemm_ozaki with the arguments of
  * TMatrixBlockedMultfunc (src/MatrixASMStubSwitch.pas:352) plus trans and
  * slices; the library holds the workspace.  Returns the status. */

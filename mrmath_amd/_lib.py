@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libmrmath_gpu.so"
 DO_NONE, DO_ADD, DO_SUB = 0, 1, 2
 NN, TN, NT = 0, 1, 2
 SOLVE_FP64, SOLVE_FP32_IR, SOLVE_BF16_IR, SOLVE_BF16X3_IR = 0, 1, 2, 3
+OZAKI_INT8, OZAKI_BF16 = 0, 1
 
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -63,6 +64,9 @@ SIGNATURES = {
     "mrm_dev_dgemm_ozaki": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _i32, _pv, _i64,
                                    _pv]),
     "mrm_mult_ozaki": (_i32, [_i32, _pd, _i64, _pd, _pd, _i64, _i64, _i64, _i64, _i64, _i64, _i32, _i32]),
+    "mrm_ozaki_workspace_bytes_ex": (_i64, [_i64, _i64, _i64, _i32, _i32]),
+    "mrm_dev_dgemm_ozaki_ex": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _i32, _i32, _i32, _pv,
+                                      _i64, _pv]),
     "mrm_lowp_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32]),
     "mrm_dev_gemm_lowp": (_i32, [_i32, _i64, _i64, _i64, _pd, _i64, _pd, _i64, _pd, _i64, _pv, _i64, _pv]),
     "mrm_dev_lineq_solve_ir": (_i32, [_i32, _pd, _i64, _i64, _pd, _i64, _pd, _i64, _i64, _i32, C.c_double, _pv, _pv,

@@ -225,11 +225,11 @@ static int host_gemm_on(GemmCtx& cx, int trans, double* dest, int64_t destLW, co
         rc = mrm_launch_dgemm(trans, M, N, K, static_cast<const double*>(dA), w1 * 8, static_cast<const double*>(dB),
                               w2 * 8, static_cast<double*>(dC), N * 8, op, s);
     } else {
-        const size_t wb = mrm_ozaki_workspace(M, N, K, slices);
+        const size_t wb = mrm_ozaki_workspace(M, N, K, slices, MRM_OZAKI_INT8);
         if ((rc = grow_buffer(cx.bufs[3], cx.caps[3], wb, s))) return rc;
         rc = mrm_launch_dgemm_ozaki(trans, M, N, K, static_cast<const double*>(dA), w1 * 8,
                                     static_cast<const double*>(dB), w2 * 8, static_cast<double*>(dC), N * 8, op, slices,
-                                    cx.bufs[3], wb, s);
+                                    MRM_OZAKI_INT8, cx.bufs[3], wb, s);
     }
     if (rc) return rc;
     g_gemms.fetch_add(1);
@@ -374,14 +374,19 @@ int32_t mrm_dev_dgemm(int32_t trans, int64_t M, int64_t N, int64_t K, const doub
     return mrm_launch_dgemm(trans, M, N, K, A, lda, B, ldb, C, ldc, op, mrm_resolve_stream(stream));
 }
 
-int64_t mrm_ozaki_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_t slices) {
+int64_t mrm_ozaki_workspace_bytes_ex(int64_t M, int64_t N, int64_t K, int32_t slices, int32_t kind) {
     if (M < 0 || N < 0 || K < 0 || slices < 1 || slices > MRM_OZAKI_MAX_SLICES) return 0;
-    return int64_t(mrm_ozaki_workspace(M, N, K, slices));
+    if (kind != MRM_OZAKI_INT8 && kind != MRM_OZAKI_BF16) return 0;
+    return int64_t(mrm_ozaki_workspace(M, N, K, slices, kind));
 }
 
-int32_t mrm_dev_dgemm_ozaki(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
-                            const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, int32_t slices,
-                            void* work, int64_t work_bytes, void* stream) {
+int64_t mrm_ozaki_workspace_bytes(int64_t M, int64_t N, int64_t K, int32_t slices) {
+    return mrm_ozaki_workspace_bytes_ex(M, N, K, slices, MRM_OZAKI_INT8);
+}
+
+int32_t mrm_dev_dgemm_ozaki_ex(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                               const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, int32_t slices,
+                               int32_t kind, void* work, int64_t work_bytes, void* stream) {
     clear_error();
     int rc = mrm_ensure_init();
     if (rc) return rc;
@@ -390,8 +395,15 @@ int32_t mrm_dev_dgemm_ozaki(int32_t trans, int64_t M, int64_t N, int64_t K, cons
         return MRM_BAD_ARG;
     }
     if (work_bytes < 0) work_bytes = 0;
-    return mrm_launch_dgemm_ozaki(trans, M, N, K, A, lda, B, ldb, C, ldc, op, slices, work, size_t(work_bytes),
-                                  mrm_resolve_stream(stream));
+    return mrm_launch_dgemm_ozaki(trans, M, N, K, A, lda, B, ldb, C, ldc, op, slices, kind, work,
+                                  size_t(work_bytes), mrm_resolve_stream(stream));
+}
+
+int32_t mrm_dev_dgemm_ozaki(int32_t trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                            const double* B, int64_t ldb, double* C, int64_t ldc, int32_t op, int32_t slices,
+                            void* work, int64_t work_bytes, void* stream) {
+    return mrm_dev_dgemm_ozaki_ex(trans, M, N, K, A, lda, B, ldb, C, ldc, op, slices, MRM_OZAKI_INT8, work,
+                                  work_bytes, stream);
 }
 
 int32_t mrm_mult_ozaki(int32_t trans, double* dest, int64_t destLineWidth, const double* mt1, const double* mt2,

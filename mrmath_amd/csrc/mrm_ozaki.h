@@ -1,4 +1,4 @@
-// mrm_ozaki.h -- FP64 GEMM emulated with INT8 slices (internal launcher).
+// mrm_ozaki.h -- FP64 GEMM emulated with INT8 or BF16 slices (internal launcher).
 #pragma once
 
 #include "mrm_common.h"
@@ -11,6 +11,6 @@
 #define MRM_OZAKI_KCHUNK 16384
 
 int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
-                           int64_t ldb, double* C, int64_t ldc, int op, int slices, void* work, size_t work_bytes,
-                           hipStream_t stream);
-size_t mrm_ozaki_workspace(int64_t M, int64_t N, int64_t K, int slices);
+                           int64_t ldb, double* C, int64_t ldc, int op, int slices, int kind, void* work,
+                           size_t work_bytes, hipStream_t stream);
+size_t mrm_ozaki_workspace(int64_t M, int64_t N, int64_t K, int slices, int kind);

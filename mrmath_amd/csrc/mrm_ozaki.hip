@@ -36,6 +36,8 @@ namespace {
 
 typedef int mrm_i4 __attribute__((ext_vector_type(4)));
 typedef int mrm_i16 __attribute__((ext_vector_type(16)));
+typedef float mrm_f16 __attribute__((ext_vector_type(16)));
+typedef __bf16 mrm_bf8 __attribute__((ext_vector_type(8)));
 typedef const __attribute__((address_space(1))) void* gptr_t;
 typedef __attribute__((address_space(3))) void* lptr_t;
 
@@ -99,15 +101,22 @@ __global__ void __launch_bounds__(256) oz_exp_cols_kernel(const double* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Slicing: a 64 (r) x 64 (k) tile per workgroup; S[t][k/32][r][k%32] (k-block
-// pitch Rp*32 bytes, slice pitch spitch bytes).  Padding rows / k beyond R / K
+// Slicing: a 64 (r) x 64 (k) tile per workgroup; S[t][kb][r][32 bytes] with
+// kb the 32-byte k block (32 INT8 digits, or 16 BF16 digits when BF), k-block
+// pitch Rp*32 bytes, slice pitch spitch bytes.  Padding rows / k beyond R / K
 // are written as 0.  Each thread cuts 4 consecutive k of one row and stores
-// their 4 digits per slice as one 32-bit word.
+// their 4 digits per slice as one 32-bit (INT8) or 64-bit (BF16) word; a
+// digit |d| <= 127 is exact in BF16 (8 significant bits), its bits are those
+// of the FP32 value's upper half.
 // ---------------------------------------------------------------------------
+template <bool BF>
 __global__ void __launch_bounds__(256) oz_slice_kernel(const double* __restrict__ X, int64_t rs, int64_t ks, int64_t R,
                                                        int64_t K, const int* __restrict__ ev, int8_t* __restrict__ S,
                                                        int64_t Rp, int64_t spitch, int s) {
-    __shared__ __attribute__((aligned(16))) int8_t tile[MRM_OZAKI_MAX_SLICES][64][80];
+    constexpr int DB = BF ? 2 : 1;          // bytes per digit
+    constexpr int ROWB = 64 * DB + 16;      // tile row: 64 digits + padding
+    constexpr int KBLK = 64 * DB / OZ_KB;   // 32-byte k blocks per 64 k
+    __shared__ __attribute__((aligned(16))) uint8_t tile[MRM_OZAKI_MAX_SLICES][64][ROWB];
     const int64_t r0 = int64_t(blockIdx.y) * 64, k0 = int64_t(blockIdx.x) * 64;
     const bool kfast = ks == 1;
     for (int q = threadIdx.x; q < 64 * 16; q += 256) {
@@ -123,24 +132,27 @@ __global__ void __launch_bounds__(256) oz_slice_kernel(const double* __restrict_
             y[i] = (r < R && k < K) ? ldexp(X[r * rs + k * ks], -e) : 0.0;
         }
         for (int t = 0; t < s; ++t) {
-            uint32_t w = 0;
+            uint64_t w = 0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 y[i] *= 128.0;
                 const double d = trunc(y[i]);
                 y[i] -= d;
-                w |= uint32_t(uint8_t(int8_t(d))) << (8 * i);
+                if (BF) w |= uint64_t(__float_as_uint(float(d)) >> 16) << (16 * i);
+                else w |= uint64_t(uint8_t(int8_t(d))) << (8 * i);
             }
-            *reinterpret_cast<uint32_t*>(&tile[t][rr][4 * kq]) = w;
+            if (BF) *reinterpret_cast<uint64_t*>(&tile[t][rr][8 * kq]) = w;
+            else *reinterpret_cast<uint32_t*>(&tile[t][rr][4 * kq]) = uint32_t(w);
         }
     }
     __syncthreads();
-    // per slice two k blocks of 64 rows x 32 bytes, each one contiguous 2 KiB run
-    for (int q = threadIdx.x; q < s * 64 * 4; q += 256) {
-        const int t = q / 256, ch = q & 1, rr = (q / 2) % 64, kb = (q / 128) % 2;
+    // per slice KBLK k blocks of 64 rows x 32 bytes, each one contiguous 2 KiB run
+    const int64_t kb0 = k0 * DB / OZ_KB;
+    for (int q = threadIdx.x; q < s * 64 * KBLK * 2; q += 256) {
+        const int t = q / (64 * KBLK * 2), ch = q & 1, rr = (q / 2) % 64, kb = (q / 128) % KBLK;
         const int64_t r = r0 + rr;
         const int4 v = *reinterpret_cast<const int4*>(&tile[t][rr][32 * kb + 16 * ch]);
-        *reinterpret_cast<int4*>(S + t * spitch + (k0 / OZ_KB + kb) * Rp * OZ_KB + r * OZ_KB + 16 * ch) = v;
+        *reinterpret_cast<int4*>(S + t * spitch + (kb0 + kb) * Rp * OZ_KB + r * OZ_KB + 16 * ch) = v;
     }
 }
 
@@ -212,7 +224,13 @@ __device__ __forceinline__ void oz_interleave() {
     }
 }
 
-template <int D0, int ND, int NSTAGE, int NWAVE>
+// BF: the slices are BF16 digits (16 per 32-byte k block) multiplied on
+// v_mfma_f32_32x32x16_bf16 with the same fragment and result layouts as the
+// INT8 instruction; the FP32 sums of 8 K steps (128 k) are exact
+// (|sum| <= 8 * 128 * 127^2 < 2^24 for diagonals up to 7) and are added into
+// the int32 accumulators after every 8 steps, so the int32 diagonal sums, and
+// the result, equal those of the INT8 slices bit for bit.
+template <int D0, int ND, int NSTAGE, int NWAVE, bool BF>
 __global__ void __launch_bounds__(NWAVE * 64, 1)
 oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, int64_t kbpa, int64_t kbpb, int64_t spa,
                 int64_t spb, int64_t kstep0, int64_t nk, const int* __restrict__ ea, const int* __restrict__ eb,
@@ -284,6 +302,26 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
         for (int p = 0; p < PBA; ++p)
 #pragma unroll
             for (int c = 0; c < 2; ++c) acc[e][p][c] = mrm_i16{};
+    constexpr int NDF = BF ? ND : 1;        // FP32 partial sums (BF16 slices only)
+    mrm_f16 accf[NDF][PBA][2];
+#pragma unroll
+    for (int e = 0; e < NDF; ++e)
+#pragma unroll
+        for (int p = 0; p < PBA; ++p)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) accf[e][p][c] = mrm_f16{};
+    auto flush = [&]() {
+#pragma unroll
+        for (int e = 0; e < NDF; ++e)
+#pragma unroll
+            for (int p = 0; p < PBA; ++p)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) acc[e][p][c][v] += int(accf[e][p][c][v]);
+                    accf[e][p][c] = mrm_f16{};
+                }
+    };
 
     // fragment of block p (A) / c (B): lane l reads row l%32 of the block,
     // 16-byte chunk l/32
@@ -341,9 +379,15 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
 #pragma unroll
                 for (int p = 0; p < PBA; ++p)
 #pragma unroll
-                    for (int c = 0; c < 2; ++c)
-                        acc[e][p][c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[h][t][p], fb[h][u][c], acc[e][p][c],
-                                                                             0, 0, 0);
+                    for (int c = 0; c < 2; ++c) {
+                        if constexpr (BF)
+                            accf[e][p][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                __builtin_bit_cast(mrm_bf8, fa[h][t][p]), __builtin_bit_cast(mrm_bf8, fb[h][u][c]),
+                                accf[e][p][c], 0, 0, 0);
+                        else
+                            acc[e][p][c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[h][t][p], fb[h][u][c],
+                                                                                 acc[e][p][c], 0, 0, 0);
+                    }
             }
         }
     };
@@ -374,6 +418,9 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
         load_half(int((st + 1) % NSTAGE), 0);
         mma_half(1);
         oz_interleave<kRefill ? LPS : 0, R0, M1>();
+        if constexpr (BF) {
+            if ((st & 7) == 7) flush();
+        }
     };
     int64_t st = 0;
     for (; st + NSTAGE < steps; ++st) step(st, std::true_type());
@@ -382,6 +429,7 @@ oz_fused_kernel(const int8_t* __restrict__ SA, const int8_t* __restrict__ SB, in
     load_half(int((steps - 1) % NSTAGE), 1);
     mma_half(0);
     mma_half(1);
+    if constexpr (BF) flush();
 
     // C (op)= diagonal D0, then (+/-)= D0+1, ...: the roundings of one launch
     // per diagonal.  Result register v of lane l is row 8(v/4) + 4(l/32) + v%4,
@@ -422,9 +470,9 @@ constexpr int oz_fstages(int T) { return 3 * 2 * T * 128 * 32 <= 160 * 1024 ? 3 
 template <int D0, int ND>
 constexpr int oz_waves() { return (D0 == 3 && ND == 4) ? 4 : 8; }
 
-template <int D0, int ND>
+template <int D0, int ND, bool BF>
 const void* oz_fused_fn() {
-    return reinterpret_cast<const void*>(&oz_fused_kernel<D0, ND, oz_fstages(D0 + ND), oz_waves<D0, ND>()>);
+    return reinterpret_cast<const void*>(&oz_fused_kernel<D0, ND, oz_fstages(D0 + ND), oz_waves<D0, ND>(), BF>);
 }
 
 // Diagonal groups per slice count, one launch each.  A group of ND diagonals
@@ -452,13 +500,46 @@ constexpr OzGroup OZ_GROUPS[MRM_OZAKI_MAX_SLICES + 1][OZ_MAX_GROUPS] = {
 };
 static_assert(MRM_OZAKI_MAX_SLICES == 8, "a group table row per slice count");
 
+// BF16 slices: a diagonal holds an int32 and an FP32 accumulator set, so the
+// groups have at most two diagonals (three spill, and so does (6,2) in its K
+// loop: s = 8 ends with (6,1) and (7,1)).
+constexpr int OZ_MAX_GROUPS_BF = 5;
+constexpr OzGroup OZ_GROUPS_BF[MRM_OZAKI_MAX_SLICES + 1][OZ_MAX_GROUPS_BF] = {
+    {},
+    {{0, 1}},
+    {{0, 2}},
+    {{0, 2}, {2, 1}},
+    {{0, 2}, {2, 2}},
+    {{0, 2}, {2, 2}, {4, 1}},
+    {{0, 2}, {2, 2}, {4, 2}},
+    {{0, 2}, {2, 2}, {4, 2}, {6, 1}},
+    {{0, 2}, {2, 2}, {4, 2}, {6, 1}, {7, 1}},
+};
+
 // The kernel of group (d0, nd) and its workgroup size.
-const void* oz_fused_for(int d0, int nd, int& threads) {
+const void* oz_fused_for(int d0, int nd, bool bf, int& threads) {
+    if (bf) {
+        threads = 512;
+        switch (d0 * 8 + nd) {
+#define OZ_CASE(a, b) \
+    case a * 8 + b: return oz_fused_fn<a, b, true>();
+            OZ_CASE(0, 1)
+            OZ_CASE(0, 2)
+            OZ_CASE(2, 1)
+            OZ_CASE(2, 2)
+            OZ_CASE(4, 1)
+            OZ_CASE(4, 2)
+            OZ_CASE(6, 1)
+            OZ_CASE(7, 1)
+#undef OZ_CASE
+            default: return nullptr;
+        }
+    }
     switch (d0 * 8 + nd) {
 #define OZ_CASE(a, b)                      \
     case a * 8 + b:                        \
         threads = 64 * oz_waves<a, b>();   \
-        return oz_fused_fn<a, b>();
+        return oz_fused_fn<a, b, false>();
         OZ_CASE(0, 1)
         OZ_CASE(0, 2)
         OZ_CASE(0, 3)
@@ -478,16 +559,23 @@ int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 }  // namespace
 
-// C (M x N) (op)= op(A) op(B) in FP64 through `slices` int8 slices per operand.
-// Device pointers, byte strides; enqueued on `stream`; `work` holds
-// mrm_ozaki_workspace(M, N, K, slices) bytes.
+// C (M x N) (op)= op(A) op(B) in FP64 through `slices` INT8 (kind
+// MRM_OZAKI_INT8) or BF16 (MRM_OZAKI_BF16) slices per operand.  Device
+// pointers, byte strides; enqueued on `stream`; `work` holds
+// mrm_ozaki_workspace(M, N, K, slices, kind) bytes.
 int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const double* B,
-                           int64_t ldb, double* C, int64_t ldc, int op, int slices, void* work, size_t work_bytes,
-                           hipStream_t stream) {
+                           int64_t ldb, double* C, int64_t ldc, int op, int slices, int kind, void* work,
+                           size_t work_bytes, hipStream_t stream) {
     if (slices < 1 || slices > MRM_OZAKI_MAX_SLICES) {
         mrm_set_error(MRM_BAD_ARG, "ozaki: slice count %d outside 1..%d", slices, MRM_OZAKI_MAX_SLICES);
         return MRM_BAD_ARG;
     }
+    if (kind != MRM_OZAKI_INT8 && kind != MRM_OZAKI_BF16) {
+        mrm_set_error(MRM_BAD_ARG, "ozaki: slice kind %d is neither MRM_OZAKI_INT8 nor MRM_OZAKI_BF16", kind);
+        return MRM_BAD_ARG;
+    }
+    const bool bf = kind == MRM_OZAKI_BF16;
+    const int db = bf ? 2 : 1;  // bytes per digit
     if (M < 0 || N < 0 || K < 0 || op < 0 || op > 2 || trans < 0 || trans > 2 || ((lda | ldb | ldc) & 7)) {
         mrm_set_error(MRM_BAD_ARG, "ozaki: negative dimension, bad op/trans or stride not a multiple of 8 bytes");
         return MRM_BAD_ARG;
@@ -499,7 +587,7 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
         return MRM_BAD_DIMS;
     }
     const int64_t Mp = round_up(M, OZ_TILE), Np = round_up(N, OZ_TILE), Kp = round_up(K, OZ_KPAD);
-    const size_t need = mrm_ozaki_workspace(M, N, K, slices);
+    const size_t need = mrm_ozaki_workspace(M, N, K, slices, kind);
     if (!work || work_bytes < need) {
         mrm_set_error(MRM_BAD_ARG, "ozaki: workspace of %zu bytes needed, %zu given", need, work_bytes);
         return MRM_BAD_ARG;
@@ -509,8 +597,8 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
     int* ea = reinterpret_cast<int*>(w);
     int* eb = ea + Mp;
     int8_t* SA = reinterpret_cast<int8_t*>(w + round_up((Mp + Np) * 4, 256));
-    int8_t* SB = SA + slices * Mp * Kp;
-    const int64_t spa = Mp * Kp, spb = Np * Kp;
+    int8_t* SB = SA + slices * Mp * Kp * db;
+    const int64_t spa = Mp * Kp * db, spb = Np * Kp * db;
     // logical A(i, k) = A[i*ars + k*aks] and B^T(j, k) = B[j*brs + k*bks] (element strides)
     int64_t ars, aks, brs, bks;
     if (trans == MRM_TN) { ars = 1; aks = ela; } else { ars = ela; aks = 1; }
@@ -525,26 +613,30 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
     else hipLaunchKernelGGL(oz_exp_cols_kernel, dim3(unsigned((N + 63) / 64), kpieces), dim3(256), 0, stream, B, bks,
                             N, K, eb);
     // slices; padding rows up to Mp / Np and k up to Kp are written as zero
-    hipLaunchKernelGGL(oz_slice_kernel, dim3(unsigned(Kp / 64), unsigned(Mp / 64)), dim3(256), 0, stream, A, ars, aks,
-                       M, K, ea, SA, Mp, spa, slices);
-    hipLaunchKernelGGL(oz_slice_kernel, dim3(unsigned(Kp / 64), unsigned(Np / 64)), dim3(256), 0, stream, B, brs, bks,
-                       N, K, eb, SB, Np, spb, slices);
-    // per K chunk, diagonals largest first in the groups of OZ_GROUPS[slices];
-    // the very first diagonal applies op, every later one accumulates with
-    // the sign op asks for
+    auto slice = bf ? oz_slice_kernel<true> : oz_slice_kernel<false>;
+    hipLaunchKernelGGL(slice, dim3(unsigned(Kp / 64), unsigned(Mp / 64)), dim3(256), 0, stream, A, ars, aks, M, K, ea,
+                       SA, Mp, spa, slices);
+    hipLaunchKernelGGL(slice, dim3(unsigned(Kp / 64), unsigned(Np / 64)), dim3(256), 0, stream, B, brs, bks, N, K, eb,
+                       SB, Np, spb, slices);
+    // per K chunk, the diagonal groups of OZ_GROUPS[slices] (OZ_GROUPS_BF
+    // for BF16 slices) in order; the very first diagonal applies op, every
+    // later one accumulates with the sign op asks for.  A K step is one
+    // 32-byte k block: 32 INT8 or 16 BF16 digits.
     const int tiles_m = int(Mp / 128), tiles_n = int(Np / 128);
-    const int64_t ksteps = Kp / OZ_KB, chunk = MRM_OZAKI_KCHUNK / OZ_KB;
+    const int64_t ksteps = Kp * db / OZ_KB, chunk = MRM_OZAKI_KCHUNK * db / OZ_KB;
+    const OzGroup* groups = bf ? OZ_GROUPS_BF[slices] : OZ_GROUPS[slices];
+    const int max_groups = bf ? OZ_MAX_GROUPS_BF : OZ_MAX_GROUPS;
     int64_t kbpa = Mp * OZ_KB, kbpb = Np * OZ_KB;
     bool first = true;
     for (int64_t k0 = 0; k0 < ksteps; k0 += chunk) {
         int64_t nk = ksteps - k0 < chunk ? ksteps - k0 : chunk;
         int64_t kk = k0;
-        for (int g = 0; g < OZ_MAX_GROUPS && OZ_GROUPS[slices][g].nd > 0; ++g) {
-            const int d0 = OZ_GROUPS[slices][g].d0, nd = OZ_GROUPS[slices][g].nd;
+        for (int g = 0; g < max_groups && groups[g].nd > 0; ++g) {
+            const int d0 = groups[g].d0, nd = groups[g].nd;
             int o = first ? op : (op == MRM_DO_SUB ? MRM_DO_SUB : MRM_DO_ADD);
             first = false;
             int threads = 0;
-            const void* fn = oz_fused_for(d0, nd, threads);
+            const void* fn = oz_fused_for(d0, nd, bf, threads);
             if (!fn) {
                 mrm_set_error(MRM_BAD_ARG, "ozaki: no kernel for diagonals %d..%d", d0, d0 + nd - 1);
                 return MRM_BAD_ARG;
@@ -566,7 +658,8 @@ int mrm_launch_dgemm_ozaki(int trans, int64_t M, int64_t N, int64_t K, const dou
     return MRM_OK;
 }
 
-size_t mrm_ozaki_workspace(int64_t M, int64_t N, int64_t K, int slices) {
+size_t mrm_ozaki_workspace(int64_t M, int64_t N, int64_t K, int slices, int kind) {
     const int64_t Mp = round_up(M, OZ_TILE), Np = round_up(N, OZ_TILE), Kp = round_up(K, OZ_KPAD);
-    return size_t(round_up((Mp + Np) * 4, 256)) + size_t(slices) * size_t(Mp + Np) * size_t(Kp);
+    const size_t db = kind == MRM_OZAKI_BF16 ? 2 : 1;
+    return size_t(round_up((Mp + Np) * 4, 256)) + db * size_t(slices) * size_t(Mp + Np) * size_t(Kp);
 }

@@ -1,4 +1,5 @@
-"""GPU parity of the INT8-slice FP64 GEMM (mrm_dev_dgemm_ozaki / mrm_mult_ozaki).
+"""GPU parity of the INT8- and BF16-slice FP64 GEMM (mrm_dev_dgemm_ozaki[_ex] /
+mrm_mult_ozaki).
 
 Bars: bit-exact against the oracle's restatement of the same method
 (oracle.ozaki_gemm) -- every step before the FP64 accumulation is exact and the
@@ -165,3 +166,110 @@ def test_device_8192_error_per_slice(trans):
               f"{err_rows:.3e}, bar {4 * n * U:.3e}")
         assert err <= 4 * n * U
         assert err_rows <= 4 * n * U
+
+
+def _dev_ozaki(A, B, s, kind, trans=_lib.NN, C0=None, op=0):
+    """C (op)= A B through mrm_dev_dgemm_ozaki_ex on device copies of the
+    logical A (M x K) and B (K x N); returns the result on the host."""
+    import torch
+    lib = _lib.load()
+    M, K = A.shape
+    N = B.shape[1]
+    a, b = _operands(trans, A, B)
+    Ad = torch.from_numpy(a).cuda()
+    Bd = torch.from_numpy(b).cuda()
+    Cd = torch.from_numpy(C0.copy() if C0 is not None else np.zeros((M, N))).cuda()
+    wb = lib.mrm_ozaki_workspace_bytes_ex(M, N, K, s, kind)
+    W = torch.empty(max(wb, 1), dtype=torch.uint8, device="cuda")
+    rc = lib.mrm_dev_dgemm_ozaki_ex(trans, M, N, K, Ad.data_ptr(), a.shape[1] * 8, Bd.data_ptr(), b.shape[1] * 8,
+                                    Cd.data_ptr(), N * 8, op, s, kind, W.data_ptr(), wb,
+                                    torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc)
+    torch.cuda.synchronize()
+    return Cd.cpu().numpy()
+
+
+@pytest.mark.parametrize("trans", [_lib.NN, _lib.TN, _lib.NT])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (130, 257, 100), (256, 192, 320)])
+def test_bf16_slices_bit_exact_vs_oracle(trans, M, N, K):
+    """BF16 digits give the INT8 digits' exact diagonal sums, so the result
+    equals the oracle's bit for bit."""
+    A = orc.pm1_wide_matrix(K, M, 700 + M)
+    B = orc.pm1_wide_matrix(N, K, 800 + N)
+    for s in (1, 4, 6):
+        assert np.array_equal(_dev_ozaki(A, B, s, _lib.OZAKI_BF16, trans), orc.ozaki_gemm(A, B, s))
+
+
+@pytest.mark.parametrize("s", range(1, 9))
+@pytest.mark.parametrize("op", [ss.doNone, ss.doSub])
+def test_bf16_every_slice_count_bit_exact(s, op):
+    """Every BF16 group table row (up to five launches of one or two
+    diagonals), with none and with the signed accumulation of doSub."""
+    M, N, K = 200, 150, 300
+    A = orc.pm1_wide_matrix(K, M, 900 + s)
+    B = orc.pm1_wide_matrix(N, K, 1000 + s)
+    C0 = orc.pm1_matrix(N, M, 1100 + s)
+    Cg = _dev_ozaki(A, B, s, _lib.OZAKI_BF16, C0=C0, op=op)
+    assert np.array_equal(Cg, orc.ozaki_gemm(A, B, s, C0.copy(), op=op))
+
+
+def test_bf16_fp32_sums_at_their_bound():
+    """All digits 127 (entries just below the row / column scale): every FP32
+    partial sum of 8 K steps reaches (d+1) 128 127^2, up to 8 128 127^2 =
+    16516096 for the last diagonal at s = 8, just under 2^24 = 16777216; the
+    sums must still be exact (result equal to the INT8 slices' and the
+    oracle's), also over K beyond one int32 chunk (two chunks of 16384)."""
+    x = 1.0 - 2.0 ** -60
+    for M, N, K in ((64, 96, 1024), (40, 33, 16384 + 300)):
+        A = np.full((M, K), x)
+        A[1::2] *= -1.0
+        B = np.full((K, N), x)
+        B[:, ::3] *= -1.0
+        Cb = _dev_ozaki(A, B, 8, _lib.OZAKI_BF16)
+        Ci = _dev_ozaki(A, B, 8, _lib.OZAKI_INT8)
+        assert np.array_equal(Cb, Ci)
+        assert np.array_equal(Cb, orc.ozaki_gemm(A, B, 8))
+
+
+def test_bf16_device_2048_equals_int8():
+    """Device tensors at 2048^3 on the wide-range inputs: BF16 and INT8 slices
+    agree bit for bit at s = 6, and the result meets the GEMM bar."""
+    import torch
+    lib = _lib.load()
+    n = 2048
+    g = torch.Generator(device="cuda").manual_seed(77)
+    e1 = torch.randint(-20, 21, (n, n), device="cuda", generator=g).double()
+    A = (torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1) * torch.exp2(e1)
+    B = (torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1) * torch.exp2(e1.t().contiguous())
+    s_ = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for kind in (_lib.OZAKI_INT8, _lib.OZAKI_BF16):
+        wb = lib.mrm_ozaki_workspace_bytes_ex(n, n, n, 6, kind)
+        W = torch.empty(wb, dtype=torch.uint8, device="cuda")
+        Co = torch.full((n, n), float("nan"), dtype=torch.float64, device="cuda")
+        assert lib.mrm_dev_dgemm_ozaki_ex(0, n, n, n, A.data_ptr(), n * 8, B.data_ptr(), n * 8, Co.data_ptr(), n * 8,
+                                          0, 6, kind, W.data_ptr(), wb, s_) == 0
+        out[kind] = Co
+    torch.cuda.synchronize()
+    assert torch.equal(out[_lib.OZAKI_INT8], out[_lib.OZAKI_BF16])
+    Cd = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    assert lib.mrm_dev_dgemm(0, n, n, n, A.data_ptr(), n * 8, B.data_ptr(), n * 8, Cd.data_ptr(), n * 8, 0, s_) == 0
+    torch.cuda.synchronize()
+    err = float(torch.linalg.norm(out[_lib.OZAKI_BF16] - Cd)) / (float(torch.linalg.norm(A)) * float(torch.linalg.norm(B)))
+    assert err <= 4 * n * U
+
+
+def test_bf16_kind_and_workspace_checks():
+    lib = _lib.load()
+    assert lib.mrm_ozaki_workspace_bytes_ex(8192, 8192, 8192, 7, _lib.OZAKI_BF16) == \
+        2 * 8192 * 4 + 2 * 7 * 2 * 8192 * 8192
+    assert lib.mrm_ozaki_workspace_bytes_ex(8192, 8192, 8192, 7, _lib.OZAKI_INT8) == \
+        lib.mrm_ozaki_workspace_bytes(8192, 8192, 8192, 7)
+    assert lib.mrm_ozaki_workspace_bytes_ex(10, 10, 10, 3, 2) == 0
+    rc = lib.mrm_dev_dgemm_ozaki_ex(_lib.NN, 128, 128, 128, 1, 1024, 1, 1024, 1, 1024, 0, 3, 2, 1, 1 << 30, None)
+    assert rc == _lib.MRM_BAD_ARG
+    # the INT8-sized workspace is too small for BF16 slices
+    wb = lib.mrm_ozaki_workspace_bytes_ex(128, 128, 128, 3, _lib.OZAKI_INT8)
+    rc = lib.mrm_dev_dgemm_ozaki_ex(_lib.NN, 128, 128, 128, 1, 1024, 1, 1024, 1, 1024, 0, 3, _lib.OZAKI_BF16, 1, wb,
+                                    None)
+    assert rc == _lib.MRM_BAD_ARG

@@ -164,6 +164,37 @@ def test_gmres_ir_keeps_bf16_factors(n, kappa, nrhs):
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= kappa * 1e-13
 
 
+@pytest.mark.parametrize("mode,n,kappa,nrhs", [
+    (_lib.SOLVE_FP32_IR, 1000, 1e2, 4),     # plain steps only
+    (_lib.SOLVE_BF16_IR, 1000, 1e2, 4),     # the plain step stalls, GMRES finishes
+    (_lib.SOLVE_BF16X3_IR, 1000, 1e6, 4),   # GMRES with split-BF16 factors
+    (_lib.SOLVE_FP32_IR, 1000, 1e7, 2),     # GMRES with FP32 factors (kappa u_fp32 near 1)
+    (_lib.SOLVE_BF16_IR, 700, 1e4, 3),      # the low-precision attempt fails: FP64 factors answer
+])
+def test_ir_path_matches_the_oracle(mode, n, kappa, nrhs):
+    """The GPU solve takes the oracle's path (oracle/ir_gmres.py, pinned in
+    tests/test_gmres_oracle.py): the same switch to GMRES and the same
+    fallback, the same number of outer steps (+-1) and of Krylov steps (within
+    a fifth, at least 2: the factors agree to rounding, not bit for bit), and
+    the same solution to the conditioning."""
+    from oracle import ir_gmres as ig
+    A, B = _system(n, kappa, nrhs, int(n + np.log10(kappa)))
+    rc, X, st = _solve_ir_stats(mode, A, B)
+    rco, Xo, so = ig.solve_ir(A, B, mode)
+    print(f"mode={mode} n={n} kappa={kappa:g}: GPU steps={st.iterations} gmres={st.gmres} krylov={st.krylov_steps} "
+          f"fell_back={st.fell_back} (low-precision attempt {st.lowp_iterations} steps, {st.lowp_krylov_steps} "
+          f"Krylov); oracle {so}")
+    assert rc == 0 and rco == 0
+    assert st.gmres == so["gmres"] and st.fell_back == so["fell_back"]
+    assert abs(st.iterations - so["iterations"]) <= 1
+    assert abs(st.krylov_steps - so["krylov_steps"]) <= max(2, so["krylov_steps"] // 5)
+    if so["fell_back"]:
+        assert st.lowp_iterations >= 1 and st.lowp_backward_error > 1e-14
+    be = orc.backward_error(A, X, B)
+    assert st.backward_error <= 1e-14 and be <= 1e-14
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= kappa * 1e-13
+
+
 def test_gmres_ir_stats_without_a_stall():
     """A well-conditioned system with FP32 factors converges by plain
     refinement: no GMRES step, no fallback, and the statistics entry agrees

@@ -107,6 +107,20 @@ def cpu_baseline(n: int = 8192, budget_s: float = 20.0, lu_n: int = 4096, lu_rhs
         times.append(time.monotonic() - t0)
     t = statistics.median(times)
     del A, B, C
+    # one thread on a smaller product (the 8192^3 one would take about a minute)
+    n1 = 2048
+    A1 = orc.pm1_matrix(n1, n1, 8101)
+    B1 = orc.pm1_matrix(n1, n1, 8102)
+    C1 = np.zeros((n1, n1))
+    orc.fast_mult(C1, A1, B1, 1)
+    t1 = []
+    for _ in range(3):
+        t0 = time.monotonic()
+        orc.fast_mult(C1, A1, B1, 1)
+        t1.append(time.monotonic() - t0)
+    one = {"n": n1, "threads": 1, "gflops": round(2.0 * n1 ** 3 / statistics.median(t1) / 1e9, 2),
+           "sample": f"dgemm NN {n1}^3, seeds 8101/8102, median of 3 runs after 1 warm-up"}
+    del A1, B1, C1
     # LU solve: one warm-up-free run (the factorisation dominates; its run
     # time is seconds), as the reference's ThrMatrixLinEQSolve would be timed
     Al = orc.pm1_matrix(lu_n, lu_n, 4401)
@@ -124,6 +138,7 @@ def cpu_baseline(n: int = 8192, budget_s: float = 20.0, lu_n: int = 4096, lu_rhs
                   f"{len(times)} runs after 1 warm-up, {used} threads of {threads} available (ThrMatrixMultEx2 "
                   f"tiling, block 128, AVX2/FMA kernel)",
         "host": {"cpu": host_model(), "cpus_available": threads, "nproc": os.cpu_count()},
+        "one_thread": one,
         "lu_solve": {"n": lu_n, "rhs": lu_rhs, "ms": round(lu_s * 1e3, 1), "threads": threads, "rc": rc,
                      "gflops": round(lu_flops / lu_s / 1e9, 2),
                      "backward_error": orc.backward_error(Al, Xl, Bl),

@@ -15,14 +15,17 @@
 //     diagonals are accumulated in FP64 in the epilogue, largest first.
 // The INT8 products run on v_mfma_i32_32x32x32_i8 (lane l supplies row (A) /
 // column (B) l%32 and one 16-byte k chunk chosen by l/32; the result register
-// v of lane l is row 8(v/4) + 4(l/32) + v%4, column l%32).
+// v of lane l is row 8(v/4) + 4(l/32) + v%4, column l%32).  The same digits
+// held as BF16 run on v_mfma_f32_32x32x16_bf16 with the same layouts and exact
+// FP32 partial sums folded into the int32 ones (MRM_OZAKI_BF16): the same
+// result bit for bit.
 //
-// Slices are stored in 32-byte k blocks, [slice][k/32][row][k%32], K padded to
-// a multiple of 64 and rows to a multiple of the 128-row tile with zeros, for
-// A by rows and for B by columns: one K step of a 128-row panel is 4 KiB of
-// contiguous memory, and the GEMM kernel has no edge cases except in its
-// epilogue.  One launch forms the slice products of up to three diagonals
-// (oz_fused_kernel).
+// Slices are stored in 32-byte k blocks, [slice][k block][row][32 bytes] (32
+// INT8 or 16 BF16 digits a block), K padded to a multiple of 64 and rows to a
+// multiple of the 128-row tile with zeros, for A by rows and for B by columns:
+// one K step of a 128-row panel is 4 KiB of contiguous memory, and the GEMM
+// kernel has no edge cases except in its epilogue.  One launch forms the slice
+// products of up to four diagonals (oz_fused_kernel).
 
 #include <cmath>
 #include <cstdint>

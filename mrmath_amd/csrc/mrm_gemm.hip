@@ -46,6 +46,8 @@
 // are one 16-byte chunk (see frag_pair).  Both halves' fragments are read at
 // the start of a step; the second half's land while the first half computes.
 
+#include <type_traits>
+
 #include "mrm_common.h"
 
 namespace {
@@ -182,6 +184,22 @@ struct PanelMover {
 
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Instruction order of one half step: MFMA, then up to one global load and up
+// to one fragment read, repeated; the last MFMA takes what is left (issued as
+// one burst after the barrier, the loads and reads held every wave's first
+// MFMA back).
+template <int NV, int NR, int NM>
+__device__ __forceinline__ void gm_interleave() {
+    if constexpr (NM > 0) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        constexpr int V = NM == 1 ? NV : (NV > 0 ? 1 : 0);
+        constexpr int R = NM == 1 ? NR : (NR > 0 ? 1 : 0);
+        if constexpr (V > 0) __builtin_amdgcn_sched_group_barrier(0x010, V, 0);  // VMEM
+        if constexpr (R > 0) __builtin_amdgcn_sched_group_barrier(0x100, R, 0);  // DS read
+        gm_interleave<NV - V, NR - R, NM - 1>();
+    }
+}
+
 // Workgroup id -> (tile_m, tile_n).  Workgroups are dealt round-robin to the 8
 // XCDs; remap so that each XCD receives a contiguous run of logical tiles, then
 // walk that run in groups of GROUP_M tile-rows so the workgroups resident on
@@ -263,18 +281,35 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
         for (int c = 0; c < 4; ++c) acc[r][c] = mrm_d4{0.0, 0.0, 0.0, 0.0};
 
     const int64_t nk = (K + BK - 1) / BK;
-    // stage 0
+    auto stage = [&](int i) { return smem + i * 2 * PANEL; };
+    // stage 0 <- K step 0, stage 1 <- K step 1
     if (la.direct_ok()) {
-        la.issue_direct(smem);
+        la.issue_direct(stage(0));
     } else {
         la.load_regs();
-        la.store(smem);
+        la.store(stage(0));
     }
     if (lb.direct_ok()) {
-        lb.issue_direct(smem + PANEL);
+        lb.issue_direct(stage(0) + PANEL);
     } else {
         lb.load_regs();
-        lb.store(smem + PANEL);
+        lb.store(stage(0) + PANEL);
+    }
+    if (nk > 1) {
+        la.advance();
+        lb.advance();
+        if (la.direct_ok()) {
+            la.issue_direct(stage(1));
+        } else {
+            la.load_regs();
+            la.store(stage(1));
+        }
+        if (lb.direct_ok()) {
+            lb.issue_direct(stage(1) + PANEL);
+        } else {
+            lb.load_regs();
+            lb.store(stage(1) + PANEL);
+        }
     }
     wait_vmem();
     __syncthreads();
@@ -282,58 +317,80 @@ dgemm_mfma_kernel(int64_t M, int64_t N, int64_t K, const double* __restrict__ A,
     // fragment rows of this lane: A row wm*64 + 16p + li, B column wn*64 + 16c + li
     const int arow = wm * 64 + li;
     const int bcol = wn * 64 + li;
+    double2 fa[2][4], fb[2][4];
+    auto read_half = [&](const double* S, int m) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) fa[m][p] = frag_pair<AKC>(S, arow + 16 * p, m, lg);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) fb[m][c] = frag_pair<BKC>(S + PANEL, bcol + 16 * c, m, lg);
+    };
+    auto mma_half = [&](int m) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[m][p].x, fb[m][c].x, acc[p][c], 0, 0, 0);
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[m][p].y, fb[m][c].y, acc[p][c], 0, 0, 0);
+    };
+    // fragment reads per half step: a 16-byte chunk per KC pair, two 8-byte
+    // reads per RC pair
+    constexpr int RD = (AKC ? 4 : 8) + (BKC ? 4 : 8);
+    read_half(stage(0), 0);
 
-    for (int64_t t = 0; t < nk; ++t) {
+    // K step t: the first half's 32 MFMAs run while the second half's
+    // fragments are read; the barrier in the middle of the step publishes step
+    // t+1 (its loads were issued one step earlier) and frees stage t%2, which
+    // then receives step t+2 while the second half's MFMAs run and step t+1's
+    // first-half fragments are read, so the matrix pipe has work on both sides
+    // of the barrier.  `fast`: a tile whose panel moves are all direct up to
+    // step t+2 (no branch, so the loads interleave with the MFMAs); otherwise
+    // the predicated path stores its registers after the MFMAs.
+    auto step = [&](int64_t t, auto fast) {
+        constexpr bool F = decltype(fast)::value;
         const int cur = int(t & 1);
-        const bool more = (t + 1) < nk;
-        double* Ns = smem + (cur ^ 1) * 2 * PANEL;
+        read_half(stage(cur), 1);
+        mma_half(0);
+        if constexpr (F) gm_interleave<0, RD, 32>();
+        wait_vmem();
+        __syncthreads();
         bool a_regs = false, b_regs = false;
-        if (more) {
+        if constexpr (F) {
             la.advance();
             lb.advance();
-            // the predicated path (edge tiles, partial K step) runs after the
-            // compute so its staging registers are not live across it
-            if (la.direct_ok()) la.issue_direct(Ns);
+            la.issue_direct(stage(cur));
+            lb.issue_direct(stage(cur) + PANEL);
+        } else if (t + 2 < nk) {
+            la.advance();
+            lb.advance();
+            if (la.direct_ok()) la.issue_direct(stage(cur));
             else a_regs = true;
-            if (lb.direct_ok()) lb.issue_direct(Ns + PANEL);
+            if (lb.direct_ok()) lb.issue_direct(stage(cur) + PANEL);
             else b_regs = true;
         }
-        const double* As = smem + cur * 2 * PANEL;
-        const double* Bs = As + PANEL;
-        // both halves' fragments are read at once; the second half's land
-        // while the first half's 32 MFMAs run
-        double2 fa[2][4], fb[2][4];
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-#pragma unroll
-            for (int p = 0; p < 4; ++p) fa[m][p] = frag_pair<AKC>(As, arow + 16 * p, m, lg);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) fb[m][c] = frag_pair<BKC>(Bs, bcol + 16 * c, m, lg);
-        }
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[m][p].x, fb[m][c].x, acc[p][c], 0, 0, 0);
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    acc[p][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[m][p].y, fb[m][c].y, acc[p][c], 0, 0, 0);
-        }
+        if (F || t + 1 < nk) read_half(stage(cur ^ 1), 0);
+        mma_half(1);
+        if constexpr (F) gm_interleave<8, RD, 32>();
         if (a_regs) {
             la.load_regs();
-            la.store(Ns);
+            la.store(stage(cur));
         }
         if (b_regs) {
             lb.load_regs();
-            lb.store(Ns + PANEL);
+            lb.store(stage(cur) + PANEL);
         }
-        wait_vmem();
-        __syncthreads();
-    }
+    };
+    // steps whose step t+2 moves directly: every step before the last partial
+    // one of a tile with all rows present and aligned operands
+    const int64_t nfull = K / BK;
+    const bool tile_direct = VEC && la.rows_full && lb.rows_full;
+    const int64_t nfast = tile_direct && nfull > 2 ? nfull - 2 : 0;
+    int64_t t = 0;
+    for (; t < nfast; ++t) step(t, std::true_type());
+    for (; t < nk; ++t) step(t, std::false_type());
 
     // Epilogue: acc[p][c] element v of lane l is row 16p + l/16 + 4v, column
     // 16c + l%16 of the wave tile (16 lanes write 128 contiguous bytes of a

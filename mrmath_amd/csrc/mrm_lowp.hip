@@ -23,6 +23,7 @@
 // v_mfma_f32_32x32x16_bf16).
 
 #include <cstdint>
+#include <type_traits>
 
 #include "mrm_common.h"
 #include "mrm_lowp.h"
@@ -80,6 +81,20 @@ __global__ void __launch_bounds__(256) lp_pack_kernel(const double* __restrict__
 }
 
 __device__ __forceinline__ int lp_slot(int r, int c) { return c ^ ((r >> 2) & 3); }
+
+// Instruction order of one half step: MFMA, then up to one global load and up
+// to one fragment read, repeated; the last MFMA takes what is left.
+template <int NV, int NR, int NM>
+__device__ __forceinline__ void lp_interleave() {
+    if constexpr (NM > 0) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        constexpr int V = NM == 1 ? NV : (NV > 0 ? 1 : 0);
+        constexpr int R = NM == 1 ? NR : (NR > 0 ? 1 : 0);
+        if constexpr (V > 0) __builtin_amdgcn_sched_group_barrier(0x010, V, 0);  // VMEM
+        if constexpr (R > 0) __builtin_amdgcn_sched_group_barrier(0x100, R, 0);  // DS read
+        lp_interleave<NV - V, NR - R, NM - 1>();
+    }
+}
 
 // XCD-aware tile order (consecutive workgroups go to different XCDs; each XCD
 // walks a contiguous run of tiles in groups of 4 tile rows).
@@ -188,27 +203,33 @@ lp_update_kernel(const uint8_t* __restrict__ PA, const uint8_t* __restrict__ PB,
 
     // Same ring as the INT8 slice GEMM: step st in stage st % S, each step's
     // second-half fragments load under its first-half MFMAs, the next step's
-    // first half under its second half.
+    // first half (and the refill of the stage just consumed) under its second
+    // half, the loads and reads interleaved with the MFMAs.
 #pragma unroll
     for (int i = 0; i < S; ++i)
         if (i < steps) issue(i);
     wait_loads(steps - 1 < S - 1 ? steps - 1 : S - 1);
     barrier();
     load_frags(0, 0, fa[0], fb[0]);
-    for (int64_t st = 0; st + 1 < steps; ++st) {
+    constexpr int NM = PM * PN * (MODE == MRM_LOWP_FP32 ? 4 : 1);  // MFMAs per half step
+    auto step = [&](int64_t st, auto refill) {
+        constexpr bool kRefill = decltype(refill)::value;
         const int cur = int(st % S);
         __builtin_amdgcn_s_waitcnt(0xC07F);
         load_frags(cur, 1, fa[1], fb[1]);
-        __builtin_amdgcn_sched_barrier(0);
         mma(fa[0], fb[0]);
+        lp_interleave<0, PM + PN, NM>();
         const int64_t last = steps - 1 < st + S - 1 ? steps - 1 : st + S - 1;
         wait_loads(last - (st + 1));
         barrier();
-        if (st + S < steps) issue(cur);
+        if constexpr (kRefill) issue(cur);
         load_frags(int((st + 1) % S), 0, fa[0], fb[0]);
-        __builtin_amdgcn_sched_barrier(0);
         mma(fa[1], fb[1]);
-    }
+        lp_interleave<kRefill ? LA + LB : 0, PM + PN, NM>();
+    };
+    int64_t st = 0;
+    for (; st + S < steps; ++st) step(st, std::true_type());
+    for (; st + 1 < steps; ++st) step(st, std::false_type());
     __builtin_amdgcn_s_waitcnt(0xC07F);
     load_frags(int((steps - 1) % S), 1, fa[1], fb[1]);
     mma(fa[0], fb[0]);

@@ -107,6 +107,8 @@ void release_ctx(GemmCtx* c) {
 }
 
 // Releases every library-owned HIP object (see mrm_shutdown).
+void pinned_release_all();  // the pinned copy buffers (below)
+
 void release_all() {
     std::lock_guard<std::mutex> lk(g_ctx.init_mu);
     if (!g_ctx.ready.load()) return;
@@ -127,6 +129,7 @@ void release_all() {
         mrm::ir_release_workspace();
         mrm::gmres_release_workspace();
     }
+    pinned_release_all();
     (void)hipStreamDestroy(g_ctx.stream);
     g_ctx.stream = nullptr;
     // give the stream-ordered allocations of the split products back
@@ -193,11 +196,151 @@ int mrm_ensure_init() {
 hipStream_t mrm_resolve_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 // ---------------------------------------------------------------------------
+// Host <-> device copies of `rows` rows of `width` bytes (pitches in bytes).
+// A 2-D copy with pageable host memory moves one row at a time (8 MiB in rows
+// of 8 KiB: ~1 GB/s measured, against 18-34 GB/s for one 1-D copy), so rows
+// that are not contiguous in host memory are gathered (scattered) by the CPU
+// into (out of) a pinned buffer that moves as one 1-D copy; only the rows'
+// own bytes are read or written, so callers that work on disjoint tiles of
+// one matrix at the same time stay correct.  Rows not contiguous on the
+// device are packed or unpacked there by a device-side 2-D copy.  A pitched
+// host side makes the call wait for its copy.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_pin_mu;
+std::vector<std::pair<void*, size_t>> g_pin_free;  // pinned host buffers not in use
+
+void* pinned_take(size_t bytes, size_t* size) {
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (size_t i = 0; i < g_pin_free.size(); ++i)
+            if (g_pin_free[i].second >= bytes) {
+                void* p = g_pin_free[i].first;
+                *size = g_pin_free[i].second;
+                g_pin_free.erase(g_pin_free.begin() + std::ptrdiff_t(i));
+                return p;
+            }
+    }
+    void* p = nullptr;
+    const size_t want = (bytes + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    *size = want;
+    return p;
+}
+
+void pinned_give(void* p, size_t size) {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_free.emplace_back(p, size);
+}
+
+void pinned_release_all() {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    for (auto& b : g_pin_free) (void)hipHostFree(b.first);
+    g_pin_free.clear();
+}
+
+int device_pack_buffer(void** tmp, size_t bytes, hipStream_t s) {
+    if (hipMallocAsync(tmp, bytes, s) != hipSuccess) {
+        (void)hipGetLastError();
+        mrm_set_error(MRM_NO_MEMORY, "cannot allocate %zu bytes of copy staging", bytes);
+        return MRM_NO_MEMORY;
+    }
+    return MRM_OK;
+}
+}  // namespace
+
+int mrm_copy_h2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows,
+                 hipStream_t s) {
+    if (width == 0 || rows == 0) return MRM_OK;
+    const size_t bytes = width * rows;
+    if (spitch == width && dpitch == width) {
+        MRM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return MRM_OK;
+    }
+    // the packed rows on the device: dst itself, or a buffer unpacked into dst
+    void* packed = dst;
+    if (dpitch != width) {
+        int rc = device_pack_buffer(&packed, bytes, s);
+        if (rc) return rc;
+    }
+    hipError_t e = hipSuccess;
+    void* pin = nullptr;
+    size_t pin_size = 0;
+    if (spitch == width) {
+        e = hipMemcpyAsync(packed, src, bytes, hipMemcpyHostToDevice, s);
+    } else if ((pin = pinned_take(bytes, &pin_size))) {
+        for (size_t r = 0; r < rows; ++r)
+            memcpy(static_cast<char*>(pin) + r * width, static_cast<const char*>(src) + r * spitch, width);
+        e = hipMemcpyAsync(packed, pin, bytes, hipMemcpyHostToDevice, s);
+    } else {
+        e = hipMemcpy2DAsync(packed, width, src, spitch, width, rows, hipMemcpyHostToDevice, s);
+    }
+    if (e == hipSuccess && packed != dst)
+        e = hipMemcpy2DAsync(dst, dpitch, packed, width, width, rows, hipMemcpyDeviceToDevice, s);
+    if (packed != dst) {
+        const hipError_t f = hipFreeAsync(packed, s);
+        if (e == hipSuccess) e = f;
+    }
+    if (pin) {
+        // the pinned buffer goes back to the pool only after its copy
+        const hipError_t f = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = f;
+        pinned_give(pin, pin_size);
+    }
+    MRM_HIP_TRY(e);
+    return MRM_OK;
+}
+
+int mrm_copy_d2h(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows,
+                 hipStream_t s) {
+    if (width == 0 || rows == 0) return MRM_OK;
+    const size_t bytes = width * rows;
+    if (spitch == width && dpitch == width) {
+        MRM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        return MRM_OK;
+    }
+    const void* packed = src;
+    void* tmp = nullptr;
+    hipError_t e = hipSuccess;
+    if (spitch != width) {
+        int rc = device_pack_buffer(&tmp, bytes, s);
+        if (rc) return rc;
+        e = hipMemcpy2DAsync(tmp, width, src, spitch, width, rows, hipMemcpyDeviceToDevice, s);
+        packed = tmp;
+    }
+    void* pin = nullptr;
+    size_t pin_size = 0;
+    if (e == hipSuccess) {
+        if (dpitch == width) {
+            e = hipMemcpyAsync(dst, packed, bytes, hipMemcpyDeviceToHost, s);
+        } else if ((pin = pinned_take(bytes, &pin_size))) {
+            e = hipMemcpyAsync(pin, packed, bytes, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e == hipSuccess)
+                for (size_t r = 0; r < rows; ++r)
+                    memcpy(static_cast<char*>(dst) + r * dpitch, static_cast<const char*>(pin) + r * width, width);
+            pinned_give(pin, pin_size);
+        } else {
+            e = hipMemcpy2DAsync(dst, dpitch, packed, width, width, rows, hipMemcpyDeviceToHost, s);
+        }
+    }
+    if (tmp) {
+        const hipError_t f = hipFreeAsync(tmp, s);
+        if (e == hipSuccess) e = f;
+    }
+    MRM_HIP_TRY(e);
+    return MRM_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Host-side GEMM staging.
 // ---------------------------------------------------------------------------
 static int copy_in(void* dst, const void* src, int64_t lw, int64_t w, int64_t h, hipStream_t s) {
-    MRM_HIP_TRY(hipMemcpy2DAsync(dst, size_t(w) * 8, src, size_t(lw), size_t(w) * 8, size_t(h),
-                                 hipMemcpyHostToDevice, s));
+    int rc = mrm_copy_h2d(dst, size_t(w) * 8, src, size_t(lw), size_t(w) * 8, size_t(h), s);
+    if (rc) return rc;
     g_h2d.fetch_add(int64_t(w) * h * 8);
     return MRM_OK;
 }
@@ -233,8 +376,7 @@ static int host_gemm_on(GemmCtx& cx, int trans, double* dest, int64_t destLW, co
     }
     if (rc) return rc;
     g_gemms.fetch_add(1);
-    MRM_HIP_TRY(hipMemcpy2DAsync(dest, size_t(destLW), dC, size_t(N) * 8, size_t(N) * 8, size_t(M),
-                                 hipMemcpyDeviceToHost, s));
+    if ((rc = mrm_copy_d2h(dest, size_t(destLW), dC, size_t(N) * 8, size_t(N) * 8, size_t(M), s))) return rc;
     g_d2h.fetch_add(M * N * 8);
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return MRM_OK;

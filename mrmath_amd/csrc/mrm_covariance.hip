@@ -222,13 +222,13 @@ int32_t mrm_covariance(double* dest, int64_t destLineWidth, const double* A, int
     if ((rc = mrm_staging(0, size_t(width * height) * 8, &dX))) return rc;
     if ((rc = mrm_staging(2, size_t(width * width) * 8, &dC))) return rc;
     if ((rc = mrm_staging(3, L.total, &dW))) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dX, size_t(width) * 8, A, size_t(LineWidthA), size_t(width) * 8, size_t(height),
-                                 hipMemcpyHostToDevice, s));
+    if ((rc = mrm_copy_h2d(dX, size_t(width) * 8, A, size_t(LineWidthA),
+                           size_t(width) * 8, size_t(height), s))) return rc;
     rc = covariance_device(static_cast<const double*>(dX), width, width, height, static_cast<double*>(dC), width,
                            unbiased, static_cast<char*>(dW), L, s);
     if (rc) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dest, size_t(destLineWidth), dC, size_t(width) * 8, size_t(width) * 8,
-                                 size_t(width), hipMemcpyDeviceToHost, s));
+    if ((rc = mrm_copy_d2h(dest, size_t(destLineWidth), dC, size_t(width) * 8,
+                           size_t(width) * 8, size_t(width), s))) return rc;
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return MRM_OK;
 }

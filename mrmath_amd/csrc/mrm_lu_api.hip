@@ -67,9 +67,7 @@ int factor_and_status(double* dA, int64_t lda, int n, int* dindx, int* dperm, hi
 int factor_host_copy(double* dA, int64_t n, const double* A, int64_t LineWidthA, int* dindx, int* dperm,
                      hipStream_t s) {
     auto upload = [&]() -> int {
-        MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
-                                     hipMemcpyHostToDevice, s));
-        return MRM_OK;
+        return mrm_copy_h2d(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n), s);
     };
     int rc = upload();
     if (rc) return rc;
@@ -176,9 +174,8 @@ int32_t mrm_lu_decomp_inplace(double* A, int64_t LineWidthA, int64_t width, int3
     if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
     rc = factor_host_copy(static_cast<double*>(dA), n, A, LineWidthA, static_cast<int*>(dI), nullptr, s);
     if (rc != MRM_OK && rc != MRM_SINGULAR) return rc;
-    // the reference leaves A partially factored on leSingular too
-    MRM_HIP_TRY(hipMemcpy2DAsync(A, size_t(LineWidthA), dA, size_t(n) * 8, size_t(n) * 8, size_t(n),
-                                 hipMemcpyDeviceToHost, s));
+    // the reference leaves A partially factored on leSingular too (rc keeps the status)
+    if (int cr = mrm_copy_d2h(A, size_t(LineWidthA), dA, size_t(n) * 8, size_t(n) * 8, size_t(n), s)) return cr;
     MRM_HIP_TRY(hipMemcpyAsync(indx, dI, size_t(n) * 4, hipMemcpyDeviceToHost, s));
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return rc;
@@ -220,8 +217,7 @@ int32_t mrm_inverse_inplace(double* A, int64_t LineWidthA, int64_t width) {
     if ((rc = mrm::lu_solve<double>(static_cast<const double*>(dLU), n, int(n), static_cast<const int*>(dP),
                                     static_cast<const double*>(dE), n, static_cast<double*>(dX), n, int(n), s)))
         return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(A, size_t(LineWidthA), dX, size_t(n) * 8, size_t(n) * 8, size_t(n),
-                                 hipMemcpyDeviceToHost, s));
+    if ((rc = mrm_copy_d2h(A, size_t(LineWidthA), dX, size_t(n) * 8, size_t(n) * 8, size_t(n), s))) return rc;
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return MRM_OK;
 }
@@ -254,7 +250,7 @@ int32_t mrm_determinant(const double* A, int64_t LineWidthA, int64_t width, doub
     std::vector<double> diag(static_cast<size_t>(n));
     std::vector<int32_t> indx(static_cast<size_t>(n));
     // the diagonal is a strided column: one double every (n+1) elements
-    MRM_HIP_TRY(hipMemcpy2DAsync(diag.data(), 8, dLU, size_t(n + 1) * 8, 8, size_t(n), hipMemcpyDeviceToHost, s));
+    if ((rc = mrm_copy_d2h(diag.data(), 8, dLU, size_t(n + 1) * 8, 8, size_t(n), s))) return rc;
     MRM_HIP_TRY(hipMemcpyAsync(indx.data(), dI, size_t(n) * 4, hipMemcpyDeviceToHost, s));
     MRM_HIP_TRY(hipStreamSynchronize(s));
     double d = 1.0;
@@ -285,14 +281,13 @@ int32_t mrm_lu_backsubst(const double* LUDecomp, int64_t LineWidthLU, int64_t wi
     if ((rc = mrm_staging(5, size_t(n) * 4, &dP))) return rc;
     if ((rc = mrm_staging(2, size_t(n) * 8, &dB))) return rc;
     if ((rc = mrm_staging(3, size_t(n) * 8, &dX))) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dLU, size_t(n) * 8, LUDecomp, size_t(LineWidthLU), size_t(n) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
+    if ((rc = mrm_copy_h2d(dLU, size_t(n) * 8, LUDecomp, size_t(LineWidthLU), size_t(n) * 8, size_t(n), s))) return rc;
     MRM_HIP_TRY(hipMemcpyAsync(dP, perm.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
-    MRM_HIP_TRY(hipMemcpy2DAsync(dB, 8, B, size_t(LineWidthB), 8, size_t(n), hipMemcpyHostToDevice, s));
+    if ((rc = mrm_copy_h2d(dB, 8, B, size_t(LineWidthB), 8, size_t(n), s))) return rc;
     rc = mrm::lu_solve<double>(static_cast<const double*>(dLU), n, int(n), static_cast<const int*>(dP),
                                static_cast<const double*>(dB), 1, static_cast<double*>(dX), 1, 1, s);
     if (rc) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(B, size_t(LineWidthB), dX, 8, 8, size_t(n), hipMemcpyDeviceToHost, s));
+    if ((rc = mrm_copy_d2h(B, size_t(LineWidthB), dX, 8, 8, size_t(n), s))) return rc;
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return MRM_OK;
 }
@@ -328,15 +323,13 @@ int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, cons
     if ((rc = mrm_staging(5, size_t(n) * 4, &dI))) return rc;
     if ((rc = mrm_staging(6, size_t(n) * 4, &dP))) return rc;
     if ((rc = mrm_staging(7, size_t(n * m) * 8, &dD))) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
+    if ((rc = mrm_copy_h2d(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n), s))) return rc;
     auto copy_a = [&]() -> int {
         MRM_HIP_TRY(hipMemcpyAsync(dLU, dA, size_t(n * n) * 8, hipMemcpyDeviceToDevice, s));
         return MRM_OK;
     };
     if ((rc = copy_a())) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dB, size_t(m) * 8, B, size_t(LineWidthB), size_t(m) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
+    if ((rc = mrm_copy_h2d(dB, size_t(m) * 8, B, size_t(LineWidthB), size_t(m) * 8, size_t(n), s))) return rc;
     double* a = static_cast<double*>(dA);
     double* lu = static_cast<double*>(dLU);
     double* b = static_cast<double*>(dB);
@@ -372,8 +365,7 @@ int32_t mrm_lineq_solve(const double* A, int64_t LineWidthA, int64_t width, cons
         if (!rc && e != hipSuccess) MRM_HIP_TRY(e);
     }
     if (rc) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(X, size_t(LineWidthX), x, size_t(m) * 8, size_t(m) * 8, size_t(n),
-                                 hipMemcpyDeviceToHost, s));
+    if ((rc = mrm_copy_d2h(X, size_t(LineWidthX), x, size_t(m) * 8, size_t(m) * 8, size_t(n), s))) return rc;
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return MRM_OK;
 }
@@ -475,17 +467,15 @@ int32_t mrm_lineq_solve_ex(const double* A, int64_t LineWidthA, int64_t width, c
     if ((rc = mrm_staging(0, size_t(n * n) * 8, &dA))) return rc;
     if ((rc = mrm_staging(2, size_t(n * m) * 8, &dB))) return rc;
     if ((rc = mrm_staging(3, size_t(n * m) * 8, &dX))) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
-    MRM_HIP_TRY(hipMemcpy2DAsync(dB, size_t(m) * 8, B, size_t(LineWidthB), size_t(m) * 8, size_t(n),
-                                 hipMemcpyHostToDevice, s));
+    if ((rc = mrm_copy_h2d(dA, size_t(n) * 8, A, size_t(LineWidthA), size_t(n) * 8, size_t(n), s))) return rc;
+    if ((rc = mrm_copy_h2d(dB, size_t(m) * 8, B, size_t(LineWidthB), size_t(m) * 8, size_t(n), s))) return rc;
     mrm::IrStats st;
     rc = mrm::lineq_solve_ir(mode, static_cast<const double*>(dA), n, int(n), static_cast<const double*>(dB), m,
                              static_cast<double*>(dX), m, int(m), NumRefinments ? int(NumRefinments) : 50, 1e-14,
                              &st, s);
     if (rc != MRM_OK && rc != MRM_NOT_CONVERGED) return rc;
-    MRM_HIP_TRY(hipMemcpy2DAsync(X, size_t(LineWidthX), dX, size_t(m) * 8, size_t(m) * 8, size_t(n),
-                                 hipMemcpyDeviceToHost, s));
+    // rc keeps the status of the solve
+    if (int cr = mrm_copy_d2h(X, size_t(LineWidthX), dX, size_t(m) * 8, size_t(m) * 8, size_t(n), s)) return cr;
     MRM_HIP_TRY(hipStreamSynchronize(s));
     return rc;
 }

@@ -231,7 +231,14 @@ void* pinned_take(size_t bytes, size_t* size) {
     return p;
 }
 
+// Buffers up to 256 MiB go back to the pool (kept until mrm_shutdown); larger
+// ones are freed at once, so one large padded copy does not keep its pinned
+// memory.
 void pinned_give(void* p, size_t size) {
+    if (size > (size_t(256) << 20)) {
+        (void)hipHostFree(p);
+        return;
+    }
     std::lock_guard<std::mutex> lk(g_pin_mu);
     g_pin_free.emplace_back(p, size);
 }

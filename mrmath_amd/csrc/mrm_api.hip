@@ -19,6 +19,7 @@
 
 #include "mrm_common.h"
 #include "mrm_host.h"
+#include "mrm_lu.h"
 #include "mrm_ozaki.h"
 
 namespace {
@@ -130,6 +131,7 @@ void release_all() {
         mrm::gmres_release_workspace();
     }
     pinned_release_all();
+    mrm::lu_lookahead_release();
     (void)hipStreamDestroy(g_ctx.stream);
     g_ctx.stream = nullptr;
     // give the stream-ordered allocations of the split products back
@@ -181,6 +183,9 @@ int mrm_ensure_init() {
     MRM_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     g_ctx.device = dev;
     g_ctx.stream = s;
+    // the LU lookahead's CU-masked streams, before any other work of the
+    // process takes hardware queues (not available: the LU runs without)
+    (void)mrm::lu_lookahead_init();
     if (!g_ctx.atexit_registered) {
         // registered after the HIP runtime initialised, so it runs before the
         // runtime's own teardown (and a profiler's finalisation)

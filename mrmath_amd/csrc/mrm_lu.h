@@ -9,13 +9,19 @@
 
 namespace mrm {
 
-constexpr int LU_NB = 512;             // outer block: K of the trailing-update GEMM (FP64)
-constexpr int LU_NB_LOWP = 512;        // outer block of the low-precision trailing update
+constexpr int LU_NB = 256;             // outer block: K of the trailing-update GEMM (FP64)
+constexpr int LU_NB_LOWP = 256;        // outer block of the low-precision trailing update
 inline int lu_outer_block(int mode) { return mode == MRM_SOLVE_FP64 ? LU_NB : LU_NB_LOWP; }
 constexpr int LU_W = 32;               // panel width of the panel kernel
 constexpr int LU_TRSM_R = 64;          // rows per diagonal block of the blocked triangular solves
 constexpr int LU_MAX_PANEL_WG = 128;   // panel workgroups (all resident; candidates fit in LDS)
 constexpr int LU_PANEL_WG_TARGET = 32; // preferred panel workgroups before packing more rows per thread
+// Lookahead: from this order on, the panel stream applies block k to block
+// k + 1 and factors it on its own CUs while the update stream applies block k
+// to the columns beyond block k + 1.
+constexpr int LU_LA_MIN_N = 6144;
+constexpr int LU_LA_MAX_N = 32768;      // up to here a panel has at most 32 workgroups, one per panel CU
+constexpr int LU_LA_CUS_PER_XCD = 4;   // panel stream: 4 of the 32 CUs of each of the 8 XCDs
 
 // Device workspace of the factorisation (one per library instance; users are
 // serialised by the host mutex).
@@ -24,6 +30,8 @@ struct LuWork {
     void* xch = nullptr;          // pivot exchange area
     void* lowp = nullptr;         // packed operands of the low-precision trailing update
     size_t lowp_bytes = 0;
+    void* lowp_la = nullptr;      // ... of the lookahead's update of the next block (panel stream)
+    size_t lowp_la_bytes = 0;
     double* backup = nullptr;     // copy of an in-place caller's matrix for a relaunch
     size_t backup_elems = 0;
     // panel launch: 0 = plain launch when the device holds the whole grid,
@@ -32,6 +40,9 @@ struct LuWork {
     // deadline of the panel waits on a plain launch in 100 MHz wall-clock
     // ticks; 0 makes every wait give up at once (a host-forced timeout)
     uint64_t plain_timeout = 0;
+    // CUs the panel kernel may use (0: the whole device); bounds the grid of
+    // a plain launch
+    int panel_cus = 0;
 };
 
 constexpr uint64_t LU_PANEL_TIMEOUT_TICKS = 2ull * 100000000ull;  // 2 s
@@ -49,6 +60,15 @@ uint64_t lu_plain_panel_timeout();
 void lu_count_relaunch();
 
 size_t lu_xch_bytes();
+// The lookahead's two streams: the panel stream (LU_LA_CUS_PER_XCD CUs of
+// every XCD) and the update stream (the other CUs).  Created at library
+// initialisation, before other work of the process can have taken the
+// hardware queues (with queues created after a hipGraph with parallel
+// branches had run, the two streams did not overlap: n = 16384 took 186 ms
+// against 136 ms).  Returns 1 when they are ready, -1 when the device has no
+// 8 x 32 CUs.  lu_lookahead_release destroys them (library shutdown).
+int lu_lookahead_init();
+void lu_lookahead_release();
 
 // Element-stride GEMM in the arithmetic type T (double: FP64 MFMA).
 template <typename T>
@@ -77,7 +97,8 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
 
 // Packed-operand bytes of the low-precision trailing updates of an n x n LU.
 size_t lu_lowp_bytes(int n, int mode);
-// Grows ws.lowp to lu_lowp_bytes(n, mode); caller holds mrm_host_mutex().
+// Grows ws.lowp to lu_lowp_bytes(n, mode) (and ws.lowp_la for the
+// lookahead's next-block updates); caller holds mrm_host_mutex().
 int lu_lowp_reserve(LuWork& ws, int n, int mode);
 
 // X = (LU)^-1 P B for nrhs columns.  dinv (optional): the diagonal-block

@@ -6,14 +6,15 @@
 // unit-lower triangular solve LUBacksup (:138-169) and the Schur update through
 // the blocked multiply (:211-227).
 //
-// Factorisation: right-looking, blocked twice.  Outer blocks of LU_NB = 512
-// columns (in every precision) feed the trailing-update GEMM with K = 512
-// (n = 16384: 172 ms against 175 ms with 256, whose 16-step K loops left the
-// tile prologue and the read-modify-write of C a larger share); each outer
-// block is factored in panels of
+// Factorisation: right-looking, blocked twice.  Outer blocks of LU_NB = 256
+// columns (in every precision) feed the trailing-update GEMM with K = 256;
+// each outer block is factored in panels of
 // LU_W = 32 columns by the panel kernel (mrm_lu_panel.hip: its workgroups agree
 // on every pivot through flagged exchange words), with the reference's pivot
-// rule.  The pivot sequence is recorded both as indx
+// rule.  From n = LU_LA_MIN_N on, a lookahead overlaps the factorisation of
+// block k + 1 (on 32 CUs of their own) with the update of the columns beyond
+// it by block k (on the other 224): the panel chain is the critical path, and
+// the trailing updates run beside it instead of between its steps.  The pivot sequence is recorded both as indx
 // (row i was exchanged with row indx[i] at step i, the reference's format)
 // and as a permutation vector perm (row i of P*A is row perm[i] of A), which
 // turns the solve's row interchanges (mrm_lusolve.hip) into one parallel
@@ -22,6 +23,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstdint>
+#include <mutex>
 
 #include "mrm_common.h"
 #include "mrm_lowp.h"
@@ -37,24 +39,26 @@ __global__ void zero_u32_kernel(unsigned* p, int n) {
 // perm from the reference's sequential pivot list: row i of P*A is row perm[i]
 // of A.  One thread replays the n exchanges on a copy held in LDS (dynamic
 // shared memory of n ints) or, for larger n, in global memory.
-__global__ void perm_from_indx_kernel(const int* __restrict__ indx, int* __restrict__ perm, int n, int in_lds) {
-    extern __shared__ int sp[];
-    int* p = in_lds ? sp : perm;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < n; ++i) {
-            const int r = indx[i];
-            if (r != i) {
-                const int a = p[i];
-                p[i] = p[r];
-                p[r] = a;
-            }
+// perm[r] = the row that ends at position r when the interchanges
+// indx[0 .. n-1] are applied in order (as LUSwap does): run them backwards
+// from r.  One thread per position; indx passes through LDS in chunks.
+constexpr int PERM_CHUNK = 4096;
+__global__ void __launch_bounds__(256) perm_from_indx_kernel(const int* __restrict__ indx, int* __restrict__ perm,
+                                                             int n) {
+    __shared__ int c[PERM_CHUNK];
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    int pos = r;
+    for (int hi = n; hi > 0; hi -= PERM_CHUNK) {
+        const int lo = hi > PERM_CHUNK ? hi - PERM_CHUNK : 0;
+        __syncthreads();
+        for (int i = lo + int(threadIdx.x); i < hi; i += 256) c[i - lo] = indx[i];
+        __syncthreads();
+        for (int i = hi - 1; i >= lo; --i) {
+            const int t = c[i - lo];
+            pos = pos == i ? t : pos == t ? i : pos;
         }
     }
-    __syncthreads();
-    if (in_lds)
-        for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = sp[i];
+    if (r < n) perm[r] = pos;
 }
 
 __global__ void iota_kernel(int* p, int n) {
@@ -204,6 +208,119 @@ int trsm_lower_unit(const T* L, int64_t ldl, T* X, int64_t ldx, int rows, int nc
     return MRM_OK;
 }
 
+namespace {
+
+// Factors the columns J .. J + jb - 1 (rows J .. n - 1) in panels of LU_W:
+// each panel, then its interchanges on the block's other columns, the block
+// row of U right of it and the GEMM update of the block's remaining columns.
+template <typename T>
+int factor_block(T* A, int64_t lda, int n, int J, int jb, int* indx, int* perm, LuWork& ws, hipStream_t s) {
+    for (int j = J; j < J + jb; j += LU_W) {
+        const int wb = J + jb - j < LU_W ? J + jb - j : LU_W;
+        int rc = lu_panel<T>(A + int64_t(j) * lda + j, lda, n - j, wb, j, indx, perm, ws, s);
+        if (rc) return rc;
+        laswp<T>(A, lda, j, j + wb, indx, J, j, j + wb, J + jb, s);
+        if (j + wb < J + jb) {
+            rc = trsm_lower_unit<T>(A + int64_t(j) * lda + j, lda, A + int64_t(j) * lda + j + wb, lda, wb,
+                                    J + jb - j - wb, s);
+            if (rc) return rc;
+            if (j + wb < n) {
+                rc = gemm_t<T>(MRM_NN, n - j - wb, J + jb - j - wb, wb, A + int64_t(j + wb) * lda + j, lda,
+                               A + int64_t(j) * lda + j + wb, lda, A + int64_t(j + wb) * lda + j + wb, lda,
+                               MRM_DO_SUB, s);
+                if (rc) return rc;
+            }
+        }
+    }
+    return MRM_OK;
+}
+
+// U12 = L11^-1 A12 for the columns c0 .. c0 + cols - 1 right of the factored
+// block J .. J + jb - 1, then the Schur update of rows J + jb .. n - 1 of
+// those columns: A22 -= L21 U12 (src/LinAlgLU.pas:211-227), in FP64 or, for
+// the refined modes, in the low precision.
+template <typename T>
+int block_row_update(T* A, int64_t lda, int n, int J, int jb, int c0, int cols, int mode, void* work,
+                     size_t work_bytes, hipStream_t s) {
+    if (cols <= 0) return MRM_OK;
+    int rc = trsm_lower_unit<T>(A + int64_t(J) * lda + J, lda, A + int64_t(J) * lda + c0, lda, jb, cols, s);
+    if (rc || J + jb >= n) return rc;
+    const int rows = n - J - jb;
+    if (mode == MRM_SOLVE_FP64)
+        return gemm_t<T>(MRM_NN, rows, cols, jb, A + int64_t(J + jb) * lda + J, lda, A + int64_t(J) * lda + c0, lda,
+                         A + int64_t(J + jb) * lda + c0, lda, MRM_DO_SUB, s);
+    return mrm_launch_lowp_update(mode, rows, cols, jb, A + int64_t(J + jb) * lda + J, lda, A + int64_t(J) * lda + c0,
+                                  lda, A + int64_t(J + jb) * lda + c0, lda, work, work_bytes, s);
+}
+
+// Bit i of a stream's CU mask selects, in the runtime's order, either CU
+// slot i / 8 of XCD i % 8 or CU slot i % 32 of XCD i / 32: the panel mask
+// takes bits x, x + 8, ... of the 32-bit word of XCD x, which are
+// LU_LA_CUS_PER_XCD CUs of XCD x under either reading, and the update stream
+// the complement, so that each stream has CUs in every XCD (workgroups are
+// dealt to the XCDs in turn; one dealt to an XCD where its stream has no CU
+// would never start).
+struct Lookahead {
+    int state = 0;  // 0 not tried, 1 ready, -1 not available
+    hipStream_t panel = nullptr, update = nullptr;
+    hipEvent_t ev[7] = {};  // fork, block factored x2, iteration done x2, join x2
+};
+Lookahead g_la;
+std::mutex g_la_mu;
+
+void destroy_lookahead() {
+    for (hipEvent_t& e : g_la.ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    if (g_la.panel) (void)hipStreamDestroy(g_la.panel);
+    if (g_la.update) (void)hipStreamDestroy(g_la.update);
+    g_la.panel = g_la.update = nullptr;
+}
+
+}  // namespace
+
+int lu_lookahead_init() {
+    static_assert(LU_LA_CUS_PER_XCD >= 1 && LU_LA_CUS_PER_XCD <= 4, "bits x + 8 t of XCD x's word, t < 4");
+    std::lock_guard<std::mutex> lk(g_la_mu);
+    if (g_la.state) return g_la.state;
+    g_la.state = -1;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus != 256) {
+        (void)hipGetLastError();
+        return g_la.state;
+    }
+    uint32_t pm[8] = {}, um[8], got[8];
+    for (int x = 0; x < 8; ++x)
+        for (int t = 0; t < LU_LA_CUS_PER_XCD; ++t) pm[x] |= 1u << (x + 8 * t);
+    for (int w = 0; w < 8; ++w) um[w] = ~pm[w];
+    bool ok = hipExtStreamCreateWithCUMask(&g_la.panel, 8, pm) == hipSuccess &&
+              hipExtStreamCreateWithCUMask(&g_la.update, 8, um) == hipSuccess;
+    ok = ok && hipExtStreamGetCUMask(g_la.panel, 8, got) == hipSuccess;
+    for (int w = 0; ok && w < 8; ++w) ok = got[w] == pm[w];
+    ok = ok && hipExtStreamGetCUMask(g_la.update, 8, got) == hipSuccess;
+    for (int w = 0; ok && w < 8; ++w) ok = got[w] == um[w];
+    for (hipEvent_t& e : g_la.ev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        destroy_lookahead();
+        (void)hipGetLastError();
+        return g_la.state;
+    }
+    g_la.state = 1;
+    return g_la.state;
+}
+
+void lu_lookahead_release() {
+    std::lock_guard<std::mutex> lk(g_la_mu);
+    destroy_lookahead();
+    g_la.state = 0;
+}
+
+namespace {
+size_t lu_lowp_la_bytes(int n, int mode);
+}  // namespace
+
 template <typename T>
 int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode) {
     // indx starts at 0 as in the reference (src/LinAlgLU.pas:289); entries a
@@ -213,47 +330,75 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
     const int xw = int(lu_xch_bytes() / 4);
     hipLaunchKernelGGL(zero_u32_kernel, dim3(grid1(xw, 256)), dim3(256), 0, s, static_cast<unsigned*>(ws.xch), xw);
     const int nb = lu_outer_block(mode);
-    for (int J = 0; J < n; J += nb) {
-        const int jb = n - J < nb ? n - J : nb;
-        for (int j = J; j < J + jb; j += LU_W) {
-            const int wb = J + jb - j < LU_W ? J + jb - j : LU_W;
-            int rc = lu_panel<T>(A + int64_t(j) * lda + j, lda, n - j, wb, j, indx, perm, ws, s);
-            if (rc) return rc;
-            laswp<T>(A, lda, j, j + wb, indx, J, j, j + wb, J + jb, s);
-            if (j + wb < J + jb) {
-                rc = trsm_lower_unit<T>(A + int64_t(j) * lda + j, lda, A + int64_t(j) * lda + j + wb, lda, wb,
-                                        J + jb - j - wb, s);
-                if (rc) return rc;
-                if (j + wb < n) {
-                    rc = gemm_t<T>(MRM_NN, n - j - wb, J + jb - j - wb, wb, A + int64_t(j + wb) * lda + j, lda,
-                                   A + int64_t(j) * lda + j + wb, lda, A + int64_t(j + wb) * lda + j + wb, lda,
-                                   MRM_DO_SUB, s);
-                    if (rc) return rc;
-                }
+    int rc = MRM_OK;
+    const bool la_work = mode == MRM_SOLVE_FP64 || ws.lowp_la_bytes >= lu_lowp_la_bytes(n, mode);
+    if (!ws.coop && n >= LU_LA_MIN_N && n <= LU_LA_MAX_N && la_work && lu_lookahead_init() == 1) {
+        // Lookahead.  Iteration k, block k = columns J .. J1 - 1:
+        //   panel stream:  signal "block k factored"; wait until the update
+        //                  stream has finished iteration k - 1 (then every
+        //                  earlier block has reached block k + 1's columns);
+        //                  block k's interchanges, U row block and Schur update
+        //                  on block k + 1's columns; factor block k + 1
+        //   update stream: wait "block k factored"; block k's interchanges on
+        //                  the columns left of it and beyond block k + 1, its U
+        //                  row block and Schur update beyond block k + 1;
+        //                  signal "iteration k done"
+        // The streams never write the same columns between two signals, and
+        // every column receives the sequential schedule's operations in its
+        // order; only the column split of the updates differs.
+        hipStream_t P = g_la.panel, U = g_la.update;
+        hipEvent_t* ev = g_la.ev;  // 0 fork, 1-2 block factored, 3-4 iteration done, 5-6 join
+        ws.panel_cus = 8 * LU_LA_CUS_PER_XCD;
+        // signal: record e on `from`, make `to` wait for it
+        auto sig = [&rc](hipEvent_t e, hipStream_t from, hipStream_t to) {
+            hipError_t r = hipEventRecord(e, from);
+            if (r == hipSuccess) r = hipStreamWaitEvent(to, e, 0);
+            if (r != hipSuccess && !rc) {
+                mrm_set_error(MRM_HIP_ERROR, "lu lookahead: stream signal: %s", hipGetErrorString(r));
+                rc = MRM_HIP_ERROR;
             }
+            return rc == MRM_OK;
+        };
+        if (sig(ev[0], s, P) && sig(ev[0], s, U))
+            rc = factor_block<T>(A, lda, n, 0, n < nb ? n : nb, indx, perm, ws, P);
+        for (int J = 0, k = 0; !rc && J < n; J += nb, ++k) {
+            const int jb = n - J < nb ? n - J : nb;
+            const int J1 = J + jb, jb1 = n - J1 < nb ? n - J1 : nb, J2 = J1 + jb1;
+            if (!sig(ev[1 + (k & 1)], P, U)) break;  // block k factored
+            laswp<T>(A, lda, J, J1, indx, 0, J, J2, n, U);
+            if ((rc = block_row_update<T>(A, lda, n, J, jb, J2, n - J2, mode, ws.lowp, ws.lowp_bytes, U))) break;
+            if (J1 >= n) break;
+            if (hipEventRecord(ev[3 + (k & 1)], U) != hipSuccess) {
+                mrm_set_error(MRM_HIP_ERROR, "lu lookahead: event record failed");
+                rc = MRM_HIP_ERROR;
+                break;
+            }
+            if (k > 0 && hipStreamWaitEvent(P, ev[3 + ((k - 1) & 1)], 0) != hipSuccess) {
+                mrm_set_error(MRM_HIP_ERROR, "lu lookahead: stream wait failed");
+                rc = MRM_HIP_ERROR;
+                break;
+            }
+            laswp<T>(A, lda, J, J1, indx, J1, J2, n, n, P);
+            if ((rc = block_row_update<T>(A, lda, n, J, jb, J1, jb1, mode, ws.lowp_la, ws.lowp_la_bytes, P))) break;
+            if ((rc = factor_block<T>(A, lda, n, J1, jb1, indx, perm, ws, P))) break;
         }
-        laswp<T>(A, lda, J, J + jb, indx, 0, J, J + jb, n, s);
-        if (J + jb < n) {
-            int rc = trsm_lower_unit<T>(A + int64_t(J) * lda + J, lda, A + int64_t(J) * lda + J + jb, lda, jb,
-                                        n - J - jb, s);
-            if (rc) return rc;
-            if (mode == MRM_SOLVE_FP64)
-                rc = gemm_t<T>(MRM_NN, n - J - jb, n - J - jb, jb, A + int64_t(J + jb) * lda + J, lda,
-                               A + int64_t(J) * lda + J + jb, lda, A + int64_t(J + jb) * lda + J + jb, lda,
-                               MRM_DO_SUB, s);
-            else
-                rc = mrm_launch_lowp_update(mode, n - J - jb, n - J - jb, jb, A + int64_t(J + jb) * lda + J, lda,
-                                            A + int64_t(J) * lda + J + jb, lda, A + int64_t(J + jb) * lda + J + jb,
-                                            lda, ws.lowp, ws.lowp_bytes, s);
-            if (rc) return rc;
+        ws.panel_cus = 0;
+        // join, also after an error: nothing of this factorisation may run
+        // after the caller's stream has moved on
+        sig(ev[5], P, s);
+        sig(ev[6], U, s);
+        if (rc) return rc;
+    } else {
+        for (int J = 0; J < n; J += nb) {
+            const int jb = n - J < nb ? n - J : nb;
+            if ((rc = factor_block<T>(A, lda, n, J, jb, indx, perm, ws, s))) return rc;
+            laswp<T>(A, lda, J, J + jb, indx, 0, J, J + jb, n, s);
+            if ((rc = block_row_update<T>(A, lda, n, J, jb, J + jb, n - J - jb, mode, ws.lowp, ws.lowp_bytes, s)))
+                return rc;
         }
     }
-    if (perm) {
-        // the replay's copy in LDS up to 64 KiB (16384 rows), in global memory beyond
-        const int in_lds = n <= 16384;
-        hipLaunchKernelGGL(perm_from_indx_kernel, dim3(1), dim3(256), in_lds ? size_t(n) * sizeof(int) : 0, s, indx,
-                           perm, n, in_lds);
-    }
+    if (perm && n > 0)
+        hipLaunchKernelGGL(perm_from_indx_kernel, dim3((n + 255) / 256), dim3(256), 0, s, indx, perm, n);
     MRM_HIP_TRY(hipGetLastError());
     return MRM_OK;
 }
@@ -266,18 +411,30 @@ size_t lu_lowp_bytes(int n, int mode) {
     return mrm_lowp_workspace(n - nb, n - nb, nb, mode);
 }
 
-int lu_lowp_reserve(LuWork& ws, int n, int mode) {
-    const size_t need = lu_lowp_bytes(n, mode);
-    if (need <= ws.lowp_bytes) return MRM_OK;
-    if (ws.lowp) MRM_HIP_TRY(hipFree(ws.lowp));
-    ws.lowp = nullptr;
-    ws.lowp_bytes = 0;
-    if (hipMalloc(&ws.lowp, need) != hipSuccess) {
+namespace {
+int grow(void*& p, size_t& have, size_t need) {
+    if (need <= have) return MRM_OK;
+    if (p) MRM_HIP_TRY(hipFree(p));
+    p = nullptr;
+    have = 0;
+    if (hipMalloc(&p, need) != hipSuccess) {
         mrm_set_error(MRM_NO_MEMORY, "lu: cannot allocate %zu bytes of packed-operand workspace", need);
         return MRM_NO_MEMORY;
     }
-    ws.lowp_bytes = need;
+    have = need;
     return MRM_OK;
+}
+// the lookahead's update of one outer block by the previous one
+size_t lu_lowp_la_bytes(int n, int mode) {
+    const int nb = lu_outer_block(mode);
+    if (mode == MRM_SOLVE_FP64 || n <= nb || n < LU_LA_MIN_N || n > LU_LA_MAX_N) return 0;
+    return mrm_lowp_workspace(n - nb, nb, nb, mode);
+}
+}  // namespace
+
+int lu_lowp_reserve(LuWork& ws, int n, int mode) {
+    int rc = grow(ws.lowp, ws.lowp_bytes, lu_lowp_bytes(n, mode));
+    return rc ? rc : grow(ws.lowp_la, ws.lowp_la_bytes, lu_lowp_la_bytes(n, mode));
 }
 
 template <typename T>

@@ -44,8 +44,10 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
     // it with ws.coop set: every panel then takes the cooperative launch,
     // which guarantees co-residency.  Larger grids are always cooperative.
     // (Callers hold the host mutex.)
+    // On the lookahead's panel stream only ws.panel_cus CUs are usable, and
+    // there the launch is always plain.
     constexpr int MAX_DEV = 64;
-    static int capacity[MAX_DEV][3];
+    static int per_cu_of[MAX_DEV][3], cus_of[MAX_DEV];
     static bool known[MAX_DEV][3];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV) dev = -1;
@@ -56,10 +58,15 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
             if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
                 hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, PANEL_THREADS, 0) != hipSuccess)
                 per_cu = cus = 0;
-            capacity[dev][ri] = per_cu * cus;
+            per_cu_of[dev][ri] = per_cu;
+            cus_of[dev] = cus;
             known[dev][ri] = true;
         }
-        cap = capacity[dev][ri];
+        cap = per_cu_of[dev][ri] * (ws.panel_cus ? ws.panel_cus : cus_of[dev]);
+    }
+    if (ws.panel_cus && (ws.coop || G > cap)) {
+        mrm_set_error(MRM_BAD_DIMS, "lu panel: %d workgroups do not fit the lookahead's %d CUs", G, ws.panel_cus);
+        return MRM_BAD_DIMS;
     }
     if (!ws.coop && G <= cap)
         e = hipLaunchKernel(fn, dim3(G), dim3(PANEL_THREADS), args, 0, s);

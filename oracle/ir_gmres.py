@@ -5,7 +5,7 @@ TEST INFRASTRUCTURE ONLY: a checker for the GPU library; only tests/ may
 import it.  numpy / scipy, FP64 unless stated.
 
 The steps, as the GPU runs them:
-  factorisation  right-looking LU with partial pivoting in outer blocks of 512
+  factorisation  right-looking LU with partial pivoting in outer blocks of 256
                  columns: each block's columns factored in FP64 with the
                  reference's pivot rule (first maximum of |a| in the current
                  row order, singular at |pivot| <= 10/MaxDouble, reciprocal
@@ -42,7 +42,7 @@ from scipy.linalg import solve_triangular
 
 SOLVE_FP64, SOLVE_FP32_IR, SOLVE_BF16_IR, SOLVE_BF16X3_IR = 0, 1, 2, 3
 OK, SINGULAR, NOT_CONVERGED = 0, 1, 2
-NB_LOWP = 512          # outer block of the low-precision factorisation
+NB_LOWP = 256          # outer block of the low-precision factorisation
 GMRES_DIM = 40         # Krylov dimension per correction
 GMRES_TOL = 1e-8       # relative tolerance on the preconditioned residual
 GMRES_PROBE = 10       # steps before a correction that gains less than 10x is given up

@@ -63,8 +63,8 @@ def test_lowp_product_exact_on_small_integers_and_rounding_bound(mode):
 @pytest.mark.parametrize("n", [300, 700])
 def test_lu_factor_fp64_pivots_match_the_reference_lu(n):
     """FP64 factors: P A = L U to rounding, |L| <= 1, and the pivot sequence of
-    the oracle's restatement of the reference's recursive LU (700 rows: two
-    outer blocks)."""
+    the oracle's restatement of the reference's recursive LU (300 rows: two
+    outer blocks, 700 rows: three)."""
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n))
     LU, perm = ig.lu_factor(A, ig.SOLVE_FP64)
@@ -83,7 +83,7 @@ def test_lu_factor_fp64_pivots_match_the_reference_lu(n):
 
 def test_lu_factor_low_precision_update_is_the_only_difference():
     """With FP32 updates the factors of a 700-row matrix differ from the FP64
-    ones by the order of 2^-24 (the first 512 columns not at all: their block
+    ones by the order of 2^-24 (the first 256 columns not at all: their block
     is factored in FP64), and reproduce A to that order."""
     rng = np.random.default_rng(7)
     n = 700
@@ -91,7 +91,8 @@ def test_lu_factor_low_precision_update_is_the_only_difference():
     LU64, p64 = ig.lu_factor(A, ig.SOLVE_FP64)
     LU32, p32 = ig.lu_factor(A, ig.SOLVE_FP32_IR)
     assert np.array_equal(p64, p32)
-    assert np.array_equal(LU64[:, :512], LU32[:, :512])
+    assert np.array_equal(LU64[:, :ig.NB_LOWP], LU32[:, :ig.NB_LOWP])
+    assert not np.array_equal(LU64[:, ig.NB_LOWP:], LU32[:, ig.NB_LOWP:])
     d = np.max(np.abs(LU64 - LU32)) / np.max(np.abs(LU64))
     assert 2.0 ** -40 < d < 2.0 ** -18
 

@@ -245,6 +245,51 @@ def test_panel_timeout_relaunches_cooperatively():
     assert lib.mrm_lu_panel_test_deadline(-1) == before + 3
 
 
+@pytest.mark.parametrize("n", [6144, 7000])
+def test_lookahead_matches_the_sequential_schedule(n):
+    """From n = 6144 on, outer block k + 1 is updated by block k and factored
+    on its own CUs while the other CUs apply block k to the columns beyond
+    block k + 1.  The sequential schedule (taken by the cooperative relaunch,
+    forced from the host with deadline 0) must give the same pivots and
+    factors equal to rounding: the two schedules differ only in how the
+    trailing updates are split by column.  n = 7000 ends in a partial outer
+    block."""
+    import torch
+    lib = _lib.load()
+    g = torch.Generator(device="cuda").manual_seed(n + 1)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    A[n // 3:] *= 10.0  # pivots from the lower rows first
+    s_ = torch.cuda.current_stream().cuda_stream
+
+    def dev_factor():
+        LU = A.clone()
+        indx = torch.zeros(n, dtype=torch.int32, device="cuda")
+        perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+        rc = lib.mrm_dev_lu_factor(LU.data_ptr(), n * 8, n, indx.data_ptr(), perm.data_ptr(), s_)
+        torch.cuda.synchronize()
+        return rc, LU, indx, perm
+
+    before = lib.mrm_lu_panel_test_deadline(-1)
+    rc, LU0, indx0, perm0 = dev_factor()
+    assert rc == 0, lib.mrm_last_error_message()
+    assert lib.mrm_lu_panel_test_deadline(-1) == before
+    try:
+        lib.mrm_lu_panel_test_deadline(0)
+        rc, LU1, indx1, perm1 = dev_factor()
+    finally:
+        lib.mrm_lu_panel_test_deadline(-1)
+    assert rc == 0 and lib.mrm_lu_panel_test_deadline(-1) == before + 1
+    assert torch.equal(indx0, indx1) and torch.equal(perm0, perm1)
+    d = float((LU0 - LU1).abs().max())
+    print(f"n={n}: max |lookahead - sequential| = {d:.3e}, bit-equal: {torch.equal(LU0, LU1)}")
+    assert d <= 64 * n * U * float(A.abs().max())
+    # and the factors reproduce P A
+    cols = torch.arange(0, n, 97, device="cuda")
+    L = torch.tril(LU0, -1) + torch.eye(n, dtype=torch.float64, device="cuda")
+    r = A[perm0.long()][:, cols] - L @ torch.triu(LU0)[:, cols]
+    assert float(r.abs().max()) <= 64 * n * U * float(A.abs().max())
+
+
 def test_backsubst_single_column_matches_oracle():
     n = 200
     A = orc.pm1_matrix(n, n, 91)

@@ -312,11 +312,11 @@ typedef struct mrm_solve_stats {
 } mrm_solve_stats;
 
 /* mrm_dev_lineq_solve_ir with every statistic of the run (stats may be NULL).
- * With FP32 / BF16 factors, when plain refinement stops halving the backward
- * error the solve continues with GMRES corrections preconditioned by the same
+ * With FP32 / BF16 factors, when a plain refinement step gains less than
+ * tenfold the solve continues with GMRES corrections preconditioned by the same
  * factors (one Krylov space per right-hand side, FP64 products by A, classical
- * Gram-Schmidt twice; Carson and Higham's GMRES-IR) before falling back to an
- * FP64 factorisation.  The reference refines with the plain step only
+ * Gram-Schmidt twice; Carson and Higham's GMRES-IR); a GMRES step that also
+ * gains less than tenfold ends the attempt, and an FP64 factorisation answers.  The reference refines with the plain step only
  * (src/LinAlgLU.pas:542-586). */
 MRM_API int32_t mrm_dev_lineq_solve_ir_stats(int32_t mode, const double* A, int64_t lda, int64_t n, const double* B,
                                              int64_t ldb, double* X, int64_t ldx, int64_t nrhs, int32_t max_iter,

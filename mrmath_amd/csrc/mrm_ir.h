@@ -23,10 +23,11 @@ constexpr double IR_GMRES_TOL = 1e-8;
 
 // X (n x nrhs) = A^-1 B, all device memory, element strides.  The LU of a copy
 // of A is computed in `mode` (MRM_SOLVE_*), then refined in FP64 until the
-// normwise backward error is <= tol or max_iter steps.  When the plain step
-// stops halving the backward error with low-precision factors, the remaining
+// normwise backward error is <= tol or max_iter steps.  When a plain step
+// with low-precision factors gains less than tenfold, the remaining
 // corrections come from GMRES preconditioned by the same factors (GMRES-IR);
-// only if that stalls as well is the factorisation redone in FP64.  Synchronises `s`.  Caller holds
+// only if a GMRES step also gains less than tenfold is the factorisation
+// redone in FP64.  Synchronises `s`.  Caller holds
 // mrm_host_mutex().
 int lineq_solve_ir(int mode, const double* A, int64_t lda, int n, const double* B, int64_t ldb, double* X,
                    int64_t ldx, int nrhs, int max_iter, double tol, IrStats* stats, hipStream_t s);

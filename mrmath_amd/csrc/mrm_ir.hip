@@ -9,10 +9,10 @@
 //   max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
 // reaches the tolerance, every residual formed in FP64 from the original A:
 //   x = (LU)^-1 P b;  repeat r = b - A x;  x += (LU)^-1 P r.
-// When the plain step stops halving the backward error (kappa(A) u_f near or
-// above 1 for the factors' precision u_f), the corrections d ~= A^-1 r come
-// from GMRES preconditioned by the same factors instead (GMRES-IR,
-// mrm_gmres.hip); if that stalls too, the factorisation is redone in FP64.
+// When a plain step gains less than tenfold (kappa(A) u_f near or above 1 for
+// the factors' precision u_f), the corrections d ~= A^-1 r come from GMRES
+// preconditioned by the same factors instead (GMRES-IR, mrm_gmres.hip); if a
+// GMRES step also gains less than tenfold, the factorisation is redone in FP64.
 
 #include <cmath>
 #include <cstdint>
@@ -176,13 +176,15 @@ int factor_solve_refine(int mode, const double* A, int64_t lda, int n, const dou
         st.backward_error = berr;
         if (berr <= tol) return MRM_OK;
         if (it >= max_iter || std::isnan(berr)) return MRM_NOT_CONVERGED;
-        if (gmres ? !(berr < 0.5 * prev) : !(berr < IR_PLAIN_RATE * prev)) {
-            // the plain step contracts slowly or not at all: with
+        if (!(berr < IR_PLAIN_RATE * prev)) {
+            // the step gained less than tenfold: after a plain step with
             // low-precision factors continue with GMRES corrections
-            // preconditioned by them; a GMRES step that does not halve the
-            // backward error (or any stall in FP64) ends the run
+            // preconditioned by them; after a GMRES step (GMRES did not reach
+            // its tolerance: the factors are too poor a preconditioner, and
+            // FP64 factors are the faster way) or a stall in FP64 (less than
+            // twofold) the run ends
             if (gmres || mode == MRM_SOLVE_FP64) {
-                if (!(berr < 0.5 * prev)) return MRM_NOT_CONVERGED;
+                if (gmres || !(berr < 0.5 * prev)) return MRM_NOT_CONVERGED;
             } else {
                 gmres = true;
                 st.gmres = 1;

@@ -20,8 +20,8 @@ The steps, as the GPU runs them:
                  ||b_j||inf); plain steps x += (LU)^-1 P r (the reference's
                  MatrixLinEQSolve loop, src/LinAlgLU.pas:542-586) while each
                  gains at least tenfold, GMRES corrections after the first
-                 that does not; a GMRES step that does not halve the error ends
-                 the attempt, which is then redone with FP64 factors;
+                 that does not; a GMRES step that also gains less than tenfold
+                 ends the attempt, which is then redone with FP64 factors;
   GMRES          left-preconditioned by the factors (Carson and Higham, SISC
                  2017 and 2018): V0 = (LU)^-1 P r / beta, Arnoldi on
                  (LU)^-1 P A with classical Gram-Schmidt twice, Givens rotations
@@ -218,9 +218,9 @@ def _factor_solve_refine(A, B, mode, tol, max_iter):
             return OK, X, st
         if it >= max_iter or np.isnan(berr):
             return NOT_CONVERGED, X, st
-        if (not (berr < 0.5 * prev)) if gmres else (not (berr < PLAIN_RATE * prev)):
+        if not (berr < PLAIN_RATE * prev):
             if gmres or mode == SOLVE_FP64:
-                if not (berr < 0.5 * prev):
+                if gmres or not (berr < 0.5 * prev):
                     return NOT_CONVERGED, X, st
             else:
                 gmres = True

@@ -211,3 +211,23 @@ def test_driver_paths():
     rc, X, st = ig.solve_ir(A, B, ig.SOLVE_BF16_IR)
     assert rc == 0 and st["fell_back"] == 1
     assert orc.backward_error(A, np.ascontiguousarray(X), B) <= 1e-14
+
+
+def test_gmres_step_gaining_less_than_tenfold_ends_the_attempt(monkeypatch):
+    """A GMRES step must gain tenfold like a plain step; one that gains about
+    3x (a correction of 0.7 A^-1 r leaves 0.3 r) ends the low-precision
+    attempt after that single step, and the FP64 factors answer.  (A rule
+    that only asks each GMRES step to halve the error takes 20 such steps
+    here.)"""
+    A, B = _system(400, 1e2, 2, 9)
+    calls = []
+
+    def weak(A_, LU, perm, R, **kw):
+        calls.append(1)
+        return 0.7 * np.linalg.solve(A_, R), 5
+
+    monkeypatch.setattr(ig, "gmres_correction", weak)
+    rc, X, st = ig.solve_ir(A, B, ig.SOLVE_BF16_IR)
+    assert rc == 0 and st["fell_back"] == 1
+    assert len(calls) == 1 and st["lowp_krylov_steps"] == 5
+    assert orc.backward_error(A, np.ascontiguousarray(X), B) <= 1e-14

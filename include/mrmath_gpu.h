@@ -107,6 +107,13 @@ MRM_API void mrm_reset_stats(void);
  * number of such relaunches so far in the process. */
 MRM_API int64_t mrm_lu_panel_test_deadline(int64_t ticks);
 
+/* Query, no reference counterpart: 1 when the LU factorisation overlaps the
+ * factorisation of each outer block with the previous block's trailing
+ * update on two CU-masked streams of the library (orders 6144 to 32768 on a
+ * device of 256 CUs in 8 XCDs), 0 when it runs one kernel after the other.
+ * Initialises the library. */
+MRM_API int32_t mrm_lu_lookahead_available(void);
+
 /* ---- host-pointer GEMM entries (stub-switch surface) --------------------- */
 
 /* TMatrixMultFunc (src/MatrixASMStubSwitch.pas:351), the type of multFunc:

@@ -42,6 +42,7 @@ SIGNATURES = {
     "mrm_get_stats": (None, [C.POINTER(Stats)]),
     "mrm_reset_stats": (None, []),
     "mrm_lu_panel_test_deadline": (_i64, [_i64]),
+    "mrm_lu_lookahead_available": (_i32, []),
     "mrm_thr_mult": (None, _thr),
     "mrm_thr_mult_t1": (None, _thr),
     "mrm_thr_mult_t2": (None, _thr),

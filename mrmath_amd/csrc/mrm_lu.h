@@ -69,6 +69,8 @@ size_t lu_xch_bytes();
 // 8 x 32 CUs.  lu_lookahead_release destroys them (library shutdown).
 int lu_lookahead_init();
 void lu_lookahead_release();
+// The streams exist and belong to the current device.
+bool lu_lookahead_usable();
 
 // Element-stride GEMM in the arithmetic type T (double: FP64 MFMA).
 template <typename T>

@@ -261,7 +261,8 @@ int block_row_update(T* A, int64_t lda, int n, int J, int jb, int c0, int cols, 
 // dealt to the XCDs in turn; one dealt to an XCD where its stream has no CU
 // would never start).
 struct Lookahead {
-    int state = 0;  // 0 not tried, 1 ready, -1 not available
+    int state = 0;   // 0 not tried, 1 ready, -1 not available
+    int device = -1; // the device the streams belong to
     hipStream_t panel = nullptr, update = nullptr;
     hipEvent_t ev[7] = {};  // fork, block factored x2, iteration done x2, join x2
 };
@@ -307,14 +308,22 @@ int lu_lookahead_init() {
         (void)hipGetLastError();
         return g_la.state;
     }
+    g_la.device = dev;
     g_la.state = 1;
     return g_la.state;
+}
+
+bool lu_lookahead_usable() {
+    if (lu_lookahead_init() != 1) return false;
+    int dev = -1;
+    return hipGetDevice(&dev) == hipSuccess && dev == g_la.device;
 }
 
 void lu_lookahead_release() {
     std::lock_guard<std::mutex> lk(g_la_mu);
     destroy_lookahead();
     g_la.state = 0;
+    g_la.device = -1;
 }
 
 namespace {
@@ -332,7 +341,7 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
     const int nb = lu_outer_block(mode);
     int rc = MRM_OK;
     const bool la_work = mode == MRM_SOLVE_FP64 || ws.lowp_la_bytes >= lu_lowp_la_bytes(n, mode);
-    if (!ws.coop && n >= LU_LA_MIN_N && n <= LU_LA_MAX_N && la_work && lu_lookahead_init() == 1) {
+    if (!ws.coop && n >= LU_LA_MIN_N && n <= LU_LA_MAX_N && la_work && lu_lookahead_usable()) {
         // Lookahead.  Iteration k, block k = columns J .. J1 - 1:
         //   panel stream:  signal "block k factored"; wait until the update
         //                  stream has finished iteration k - 1 (then every

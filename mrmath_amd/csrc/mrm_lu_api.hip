@@ -107,6 +107,11 @@ int64_t mrm_lu_panel_test_deadline(int64_t ticks) {
     return mrm::g_relaunches.load();
 }
 
+int32_t mrm_lu_lookahead_available(void) {
+    if (mrm_ensure_init()) return 0;
+    return mrm::lu_lookahead_usable() ? 1 : 0;
+}
+
 int32_t mrm_dev_lu_factor(double* A, int64_t lda, int64_t n, int32_t* indx, int32_t* perm, void* stream) {
     int rc = mrm_ensure_init();
     if (rc) return rc;

@@ -269,6 +269,8 @@ def test_lookahead_matches_the_sequential_schedule(n):
         torch.cuda.synchronize()
         return rc, LU, indx, perm
 
+    # the MI355X (256 CUs in 8 XCDs) has the lookahead's streams
+    assert lib.mrm_lu_lookahead_available() == 1
     before = lib.mrm_lu_panel_test_deadline(-1)
     rc, LU0, indx0, perm0 = dev_factor()
     assert rc == 0, lib.mrm_last_error_message()

@@ -127,11 +127,11 @@ void release_all() {
             g_caps[i] = 0;
         }
         mrm::lu_release_workspace();
+        mrm::lu_lookahead_release();  // the LU entries use its streams under this mutex
         mrm::ir_release_workspace();
         mrm::gmres_release_workspace();
     }
     pinned_release_all();
-    mrm::lu_lookahead_release();
     (void)hipStreamDestroy(g_ctx.stream);
     g_ctx.stream = nullptr;
     // give the stream-ordered allocations of the split products back

@@ -292,6 +292,28 @@ def test_lookahead_matches_the_sequential_schedule(n):
     assert float(r.abs().max()) <= 64 * n * U * float(A.abs().max())
 
 
+def test_lookahead_after_shutdown_and_restart():
+    """mrm_shutdown frees the lookahead's streams with the rest of the
+    library; the next call sets them up again and the factorisation repeats
+    bit for bit."""
+    import torch
+    lib = _lib.load()
+    n = 6144
+    g = torch.Generator(device="cuda").manual_seed(61)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    s_ = torch.cuda.current_stream().cuda_stream
+    out = []
+    for _ in range(2):
+        assert lib.mrm_lu_lookahead_available() == 1
+        LU = A.clone()
+        indx = torch.zeros(n, dtype=torch.int32, device="cuda")
+        assert lib.mrm_dev_lu_factor(LU.data_ptr(), n * 8, n, indx.data_ptr(), 0, s_) == 0
+        torch.cuda.synchronize()
+        out.append((LU, indx))
+        lib.mrm_shutdown()
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
 def test_backsubst_single_column_matches_oracle():
     n = 200
     A = orc.pm1_matrix(n, n, 91)

@@ -367,7 +367,9 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
     device entry (factorisation, solves, residuals, norms): one untimed call
     per mode first (it grows the library's work buffers), then the median of
     3 calls.  The backward error is also recomputed from an independent
-    residual (torch's FP64 matmul)."""
+    residual (torch's FP64 matmul).  At kappa 1e2 also the FP64 factorisation
+    alone (mrm_dev_lu_factor on a copy of A, median of 3 after one untimed
+    call), with whether it ran the lookahead on CU-masked streams."""
     import ctypes as C
     import torch
 
@@ -412,6 +414,24 @@ def lu_ir_row(lib, n: int = 16384, nrhs: int = 64):
         for name in ("fp32_ir", "bf16_ir", "bf16x3_ir"):
             res[name]["speedup_vs_fp64"] = round(res["fp64"]["ms"] / res[name]["ms"], 3)
         out["kappa"][f"{kappa:g}"] = res
+        if kappa == 1e2:
+            # the FP64 factorisation alone (mrm_dev_lu_factor on a copy of A)
+            LU = torch.empty_like(A)
+            indx = torch.empty(n, dtype=torch.int32, device=dev)
+            perm = torch.empty(n, dtype=torch.int32, device=dev)
+            ts = []
+            for rep in range(4):
+                LU.copy_(A)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                rc = lib.mrm_dev_lu_factor(LU.data_ptr(), n * 8, n, indx.data_ptr(), perm.data_ptr(), sp)
+                torch.cuda.synchronize()
+                if rep > 0:
+                    ts.append(time.perf_counter() - t)
+            ms = 1e3 * statistics.median(ts)
+            out["fp64_factor"] = {"rc": rc, "ms": round(ms, 1), "tflops": round(2 * n ** 3 / 3 / ms / 1e9, 2),
+                                  "lookahead": bool(lib.mrm_lu_lookahead_available())}
+            del LU
         del A
     return out
 

@@ -16,9 +16,9 @@ constexpr int LU_W = 32;               // panel width of the panel kernel
 constexpr int LU_TRSM_R = 64;          // rows per diagonal block of the blocked triangular solves
 constexpr int LU_MAX_PANEL_WG = 128;   // panel workgroups (all resident; candidates fit in LDS)
 constexpr int LU_PANEL_WG_TARGET = 32; // preferred panel workgroups before packing more rows per thread
-// Lookahead: from this order on, the panel stream applies block k to block
-// k + 1 and factors it on its own CUs while the update stream applies block k
-// to the columns beyond block k + 1.
+// Lookahead: from this order on, the panel stream factors block k + 1 on its
+// own CUs while the update stream applies block k to the columns beyond it
+// (after applying it to block k + 1 first).
 constexpr int LU_LA_MIN_N = 6144;
 constexpr int LU_LA_MAX_N = 32768;      // up to here a panel has at most 32 workgroups, one per panel CU
 constexpr int LU_LA_CUS_PER_XCD = 4;   // panel stream: 4 of the 32 CUs of each of the 8 XCDs
@@ -30,8 +30,6 @@ struct LuWork {
     void* xch = nullptr;          // pivot exchange area
     void* lowp = nullptr;         // packed operands of the low-precision trailing update
     size_t lowp_bytes = 0;
-    void* lowp_la = nullptr;      // ... of the lookahead's update of the next block (panel stream)
-    size_t lowp_la_bytes = 0;
     double* backup = nullptr;     // copy of an in-place caller's matrix for a relaunch
     size_t backup_elems = 0;
     // panel launch: 0 = plain launch when the device holds the whole grid,
@@ -99,8 +97,7 @@ int lu_panel(T* P, int64_t lda, int m, int w, int piv_base, int* indx, int* perm
 
 // Packed-operand bytes of the low-precision trailing updates of an n x n LU.
 size_t lu_lowp_bytes(int n, int mode);
-// Grows ws.lowp to lu_lowp_bytes(n, mode) (and ws.lowp_la for the
-// lookahead's next-block updates); caller holds mrm_host_mutex().
+// Grows ws.lowp to lu_lowp_bytes(n, mode); caller holds mrm_host_mutex().
 int lu_lowp_reserve(LuWork& ws, int n, int mode);
 
 // X = (LU)^-1 P B for nrhs columns.  dinv (optional): the diagonal-block

@@ -264,7 +264,7 @@ struct Lookahead {
     int state = 0;   // 0 not tried, 1 ready, -1 not available
     int device = -1; // the device the streams belong to
     hipStream_t panel = nullptr, update = nullptr;
-    hipEvent_t ev[7] = {};  // fork, block factored x2, iteration done x2, join x2
+    hipEvent_t ev[7] = {};  // fork, block factored x2, block ready x2, join x2
 };
 Lookahead g_la;
 std::mutex g_la_mu;
@@ -326,10 +326,6 @@ void lu_lookahead_release() {
     g_la.device = -1;
 }
 
-namespace {
-size_t lu_lowp_la_bytes(int n, int mode);
-}  // namespace
-
 template <typename T>
 int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStream_t s, int mode) {
     // indx starts at 0 as in the reference (src/LinAlgLU.pas:289); entries a
@@ -340,23 +336,20 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
     hipLaunchKernelGGL(zero_u32_kernel, dim3(grid1(xw, 256)), dim3(256), 0, s, static_cast<unsigned*>(ws.xch), xw);
     const int nb = lu_outer_block(mode);
     int rc = MRM_OK;
-    const bool la_work = mode == MRM_SOLVE_FP64 || ws.lowp_la_bytes >= lu_lowp_la_bytes(n, mode);
-    if (!ws.coop && n >= LU_LA_MIN_N && n <= LU_LA_MAX_N && la_work && lu_lookahead_usable()) {
-        // Lookahead.  Iteration k, block k = columns J .. J1 - 1:
-        //   panel stream:  signal "block k factored"; wait until the update
-        //                  stream has finished iteration k - 1 (then every
-        //                  earlier block has reached block k + 1's columns);
-        //                  block k's interchanges, U row block and Schur update
-        //                  on block k + 1's columns; factor block k + 1
-        //   update stream: wait "block k factored"; block k's interchanges on
-        //                  the columns left of it and beyond block k + 1, its U
-        //                  row block and Schur update beyond block k + 1;
-        //                  signal "iteration k done"
+    if (!ws.coop && n >= LU_LA_MIN_N && n <= LU_LA_MAX_N && lu_lookahead_usable()) {
+        // Lookahead.  Iteration k, block k = columns J .. J1 - 1, block k + 1 =
+        // J1 .. J2 - 1:
+        //   update stream (224 CUs): wait "block k factored"; block k's
+        //       interchanges, U row block and Schur update on block k + 1's
+        //       columns; signal "block k + 1 ready"; then the same on every
+        //       column left of block k and from J2 on
+        //   panel stream (32 CUs): wait "block k + 1 ready"; factor block
+        //       k + 1; signal "block k + 1 factored"
         // The streams never write the same columns between two signals, and
         // every column receives the sequential schedule's operations in its
-        // order; only the column split of the updates differs.
+        // order (only the column split of the updates differs).
         hipStream_t P = g_la.panel, U = g_la.update;
-        hipEvent_t* ev = g_la.ev;  // 0 fork, 1-2 block factored, 3-4 iteration done, 5-6 join
+        hipEvent_t* ev = g_la.ev;  // 0 fork, 1-2 block factored, 3-4 block ready, 5-6 join
         ws.panel_cus = 8 * LU_LA_CUS_PER_XCD;
         // signal: record e on `from`, make `to` wait for it
         auto sig = [&rc](hipEvent_t e, hipStream_t from, hipStream_t to) {
@@ -374,22 +367,16 @@ int lu_factor(T* A, int64_t lda, int n, int* indx, int* perm, LuWork& ws, hipStr
             const int jb = n - J < nb ? n - J : nb;
             const int J1 = J + jb, jb1 = n - J1 < nb ? n - J1 : nb, J2 = J1 + jb1;
             if (!sig(ev[1 + (k & 1)], P, U)) break;  // block k factored
+            if (J1 >= n) {
+                laswp<T>(A, lda, J, J1, indx, 0, J, n, n, U);
+                break;
+            }
+            laswp<T>(A, lda, J, J1, indx, J1, J2, n, n, U);
+            if ((rc = block_row_update<T>(A, lda, n, J, jb, J1, jb1, mode, ws.lowp, ws.lowp_bytes, U))) break;
+            if (!sig(ev[3 + (k & 1)], U, P)) break;  // block k + 1 ready
+            if ((rc = factor_block<T>(A, lda, n, J1, jb1, indx, perm, ws, P))) break;
             laswp<T>(A, lda, J, J1, indx, 0, J, J2, n, U);
             if ((rc = block_row_update<T>(A, lda, n, J, jb, J2, n - J2, mode, ws.lowp, ws.lowp_bytes, U))) break;
-            if (J1 >= n) break;
-            if (hipEventRecord(ev[3 + (k & 1)], U) != hipSuccess) {
-                mrm_set_error(MRM_HIP_ERROR, "lu lookahead: event record failed");
-                rc = MRM_HIP_ERROR;
-                break;
-            }
-            if (k > 0 && hipStreamWaitEvent(P, ev[3 + ((k - 1) & 1)], 0) != hipSuccess) {
-                mrm_set_error(MRM_HIP_ERROR, "lu lookahead: stream wait failed");
-                rc = MRM_HIP_ERROR;
-                break;
-            }
-            laswp<T>(A, lda, J, J1, indx, J1, J2, n, n, P);
-            if ((rc = block_row_update<T>(A, lda, n, J, jb, J1, jb1, mode, ws.lowp_la, ws.lowp_la_bytes, P))) break;
-            if ((rc = factor_block<T>(A, lda, n, J1, jb1, indx, perm, ws, P))) break;
         }
         ws.panel_cus = 0;
         // join, also after an error: nothing of this factorisation may run
@@ -420,30 +407,18 @@ size_t lu_lowp_bytes(int n, int mode) {
     return mrm_lowp_workspace(n - nb, n - nb, nb, mode);
 }
 
-namespace {
-int grow(void*& p, size_t& have, size_t need) {
-    if (need <= have) return MRM_OK;
-    if (p) MRM_HIP_TRY(hipFree(p));
-    p = nullptr;
-    have = 0;
-    if (hipMalloc(&p, need) != hipSuccess) {
+int lu_lowp_reserve(LuWork& ws, int n, int mode) {
+    const size_t need = lu_lowp_bytes(n, mode);
+    if (need <= ws.lowp_bytes) return MRM_OK;
+    if (ws.lowp) MRM_HIP_TRY(hipFree(ws.lowp));
+    ws.lowp = nullptr;
+    ws.lowp_bytes = 0;
+    if (hipMalloc(&ws.lowp, need) != hipSuccess) {
         mrm_set_error(MRM_NO_MEMORY, "lu: cannot allocate %zu bytes of packed-operand workspace", need);
         return MRM_NO_MEMORY;
     }
-    have = need;
+    ws.lowp_bytes = need;
     return MRM_OK;
-}
-// the lookahead's update of one outer block by the previous one
-size_t lu_lowp_la_bytes(int n, int mode) {
-    const int nb = lu_outer_block(mode);
-    if (mode == MRM_SOLVE_FP64 || n <= nb || n < LU_LA_MIN_N || n > LU_LA_MAX_N) return 0;
-    return mrm_lowp_workspace(n - nb, nb, nb, mode);
-}
-}  // namespace
-
-int lu_lowp_reserve(LuWork& ws, int n, int mode) {
-    int rc = grow(ws.lowp, ws.lowp_bytes, lu_lowp_bytes(n, mode));
-    return rc ? rc : grow(ws.lowp_la, ws.lowp_la_bytes, lu_lowp_la_bytes(n, mode));
 }
 
 template <typename T>

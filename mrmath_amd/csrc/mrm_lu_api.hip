@@ -40,7 +40,6 @@ void lu_release_workspace() {
     (void)hipFree(g_lu_ws.status);
     (void)hipFree(g_lu_ws.xch);
     if (g_lu_ws.lowp) (void)hipFree(g_lu_ws.lowp);
-    if (g_lu_ws.lowp_la) (void)hipFree(g_lu_ws.lowp_la);
     if (g_lu_ws.backup) (void)hipFree(g_lu_ws.backup);
     g_lu_ws = LuWork();
     g_lu_ready = false;

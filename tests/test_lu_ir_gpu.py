@@ -268,6 +268,44 @@ def test_rows_per_thread_4_above_16384():
     assert float(berr) <= 1e-14
 
 
+@pytest.mark.parametrize("mode", [_lib.SOLVE_FP32_IR, _lib.SOLVE_BF16X3_IR])
+def test_lookahead_lowp_matches_the_sequential_schedule(mode):
+    """n = 6144 with low-precision trailing updates: the lookahead (the next
+    block's update on the update stream, the trailing update beside the panel
+    chain) and the sequential schedule (the cooperative relaunch, forced from
+    the host with deadline 0) give the same solution and statistics bit for
+    bit."""
+    import torch
+    lib = _lib.load()
+    assert lib.mrm_lu_lookahead_available() == 1
+    n = 6144
+    g = torch.Generator(device="cuda").manual_seed(6144 + mode)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    B = torch.randn(n, 3, dtype=torch.float64, device="cuda", generator=g)
+    s_ = torch.cuda.current_stream().cuda_stream
+
+    def solve():
+        X = torch.zeros(n, 3, dtype=torch.float64, device="cuda")
+        st = _lib.SolveStats()
+        rc = lib.mrm_dev_lineq_solve_ir_stats(mode, A.data_ptr(), n * 8, n, B.data_ptr(), 24, X.data_ptr(), 24, 3,
+                                              50, 1e-14, C.byref(st), s_)
+        torch.cuda.synchronize()
+        return rc, X, (st.iterations, st.gmres, st.krylov_steps, st.fell_back, st.backward_error)
+
+    before = lib.mrm_lu_panel_test_deadline(-1)
+    rc0, X0, st0 = solve()
+    assert rc0 == 0 and lib.mrm_lu_panel_test_deadline(-1) == before
+    try:
+        lib.mrm_lu_panel_test_deadline(0)
+        rc1, X1, st1 = solve()
+    finally:
+        lib.mrm_lu_panel_test_deadline(-1)
+    assert rc1 == 0 and lib.mrm_lu_panel_test_deadline(-1) == before + 1
+    print(f"mode={mode}: {st0} / {st1}")
+    assert torch.equal(X0, X1) and st0 == st1
+    assert st0[4] <= 1e-14
+
+
 @pytest.fixture(scope="module")
 def system_16384():
     """A = U diag(s) V^T (U, V from Householder QR of Gaussian matrices, s
